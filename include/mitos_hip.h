@@ -1,0 +1,319 @@
+/*
+ * mitos_hip.h -- C ABI of libmitos_hip.so, an MI355X (gfx950) implementation
+ * of the Mitos initial-condition hot path.
+ *
+ * Part 1 keeps the reference's own call surface (names, argument meaning,
+ * return codes) so a reference caller relinks against this library on one
+ * GPU.  Every reference-named function cites the declaration it replaces.
+ * Part 2 adds the batched particle entry points that replace the reference's
+ * per-particle loops, and Part 3 the fused whole-pipeline, ensemble and
+ * adjoint entry points.
+ *
+ * Memory: grid arrays (dg->fbox, dg->box and every array argument documented
+ * as "device") live in GPU memory.  Host code reads or writes them through
+ * mitos_grid_to_host / mitos_grid_from_host.  All Part 1 and Part 2 calls are
+ * host-synchronous; they return 0 on success and a MITOS_ERR_* code
+ * otherwise (the library never calls exit()).
+ *
+ * One rank: NX = N, X0 = 0, local_size = N*N*(N/2+1).  Grid sizes with compiled
+ * FFT passes: 8, 16, 32, 64, 128, 256, 512, 1024, 1536.
+ */
+#ifndef MITOS_HIP_H
+#define MITOS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#pragma GCC visibility push(default)
+
+/* ---- types --------------------------------------------------------------- */
+
+/* FFTW's complex types (include/fft.h:23-24 includes complex.h before fftw3.h,
+ * so the reference sees double _Complex); identical layout to double[2]. */
+#if !defined(FFTW3_H)
+#ifdef __cplusplus
+typedef _Complex double fftw_complex;
+typedef _Complex float fftwf_complex;
+#else
+typedef double _Complex fftw_complex;
+typedef float _Complex fftwf_complex;
+#endif
+#endif
+
+/* MPI communicator: only stored (one rank).  MPICH's integer handle unless
+ * mpi.h was included first. */
+#if !defined(MPI_VERSION)
+typedef int MPI_Comm;
+#ifndef MPI_COMM_WORLD
+#define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#endif
+#endif
+
+/* error codes (0..2 as src/fft.c:269-277 and src/2lpt.c:69-72) */
+#define MITOS_OK 0
+#define MITOS_ERR_DIMS 1
+#define MITOS_ERR_NOT_MOMENTUM 2
+#define MITOS_ERR_DEVICE 3
+#define MITOS_ERR_UNSUPPORTED 4
+#define MITOS_ERR_ARG 5
+#define MITOS_ERR_ALLOC 6
+#define MITOS_ERR_REAL_SPACE 7
+
+/* include/distributed_grid.h:29-60 -- same fields, same order */
+struct distributed_grid {
+    int N;
+    double boxlen;
+    MPI_Comm comm;
+    char momentum_space;
+    long int NX;
+    long int X0;
+    long int local_size;
+    fftw_complex *fbox; /* device: N*N*(N/2+1) */
+    double *box;        /* device: N*N*(N+2), padded along z */
+};
+
+/* include/fft.h:31-39 */
+struct kernel {
+    double kx, ky, kz;
+    double k;
+#ifdef __cplusplus
+    _Complex double kern;
+#else
+    double _Complex kern;
+#endif
+    const void *params;
+};
+
+/* include/input.h:124-137 (fields read by kernel_power_no_transfer) */
+struct cosmology {
+    double h;
+    double n_s;
+    double A_s;
+    double k_pivot;
+    double z_ini;
+    double log_tau_ini;
+    double rho_crit;
+    double z_source;
+    double log_tau_source;
+};
+
+/* include/fft_kernels.h:189-193 */
+struct Hermite_kern_params {
+    int order;
+    int N;
+    double boxlen;
+};
+
+/* include/random_xorshift.h:11-13, include/random.h:25 */
+struct xoshiro256ss_state {
+    uint64_t s[4];
+};
+typedef struct xoshiro256ss_state rng_state;
+
+/* ======================================================================== */
+/* Part 1: reference call surface                                           */
+/* ======================================================================== */
+
+/* include/distributed_grid.h:74-77 -- device allocation of fbox and box */
+int alloc_local_grid(struct distributed_grid *dg, int N, double boxlen, MPI_Comm comm);
+int free_local_grid(struct distributed_grid *dg);
+int free_local_real_grid(struct distributed_grid *dg);
+int free_local_complex_grid(struct distributed_grid *dg);
+
+/* host access to a grid's device arrays (either pointer may be NULL) */
+int mitos_grid_to_host(const struct distributed_grid *dg, fftw_complex *fbox_host,
+                       double *box_host);
+int mitos_grid_from_host(struct distributed_grid *dg, const fftw_complex *fbox_host,
+                         const double *box_host);
+
+/* include/random.h:27-33 (host) */
+uint64_t rand_uint64(rng_state *state);
+rng_state rand_uint64_init(uint64_t seed);
+
+/* include/grf.h:30-31 -- the state advances by the 4*(N*N*(N/2+1)-1) draws
+ * the reference consumes, so later draws continue the same stream */
+int generate_complex_grf(struct distributed_grid *dg, rng_state *state);
+int enforce_hermiticity(struct distributed_grid *dg);
+
+/* include/fft.h:90-116 */
+void fft_wavevector(int x, int y, int z, int N, double delta_k, double *kx, double *ky,
+                    double *kz, double *k);
+int fft_normalize_r2c(fftw_complex *arr, int N, double boxlen);             /* device arr */
+int fft_normalize_r2c_float(fftwf_complex *arr, int N, double boxlen);      /* device arr */
+int fft_normalize_r2c_float_mpi(fftwf_complex *farr, long int N, long int local_size,
+                                double boxlen);                             /* device arr */
+int fft_normalize_c2r(double *arr, int N, double boxlen);                   /* device N^3 */
+int fft_apply_kernel(fftw_complex *write, const fftw_complex *read, int N, double boxlen,
+                     void (*compute)(struct kernel *the_kernel), const void *params);
+int fft_apply_kernel_float(fftwf_complex *write, const fftwf_complex *read, int N,
+                           double boxlen, void (*compute)(struct kernel *the_kernel),
+                           const void *params);
+int fft_apply_kernel_float_mpi(fftwf_complex *write, const fftwf_complex *read, long int N,
+                               long int X0, long int NX, double boxlen,
+                               void (*compute)(struct kernel *the_kernel),
+                               const void *params);
+int fft_r2c_dg(struct distributed_grid *dg);
+int fft_c2r_dg(struct distributed_grid *dg);
+int fft_apply_kernel_dg(struct distributed_grid *dg_write, const struct distributed_grid *dg_read,
+                        void (*compute)(struct kernel *the_kernel), const void *params);
+
+/* Kernel callbacks (include/fft_kernels.h:28-31, 96-121, 198-219).  They
+ * evaluate the kernel on the host when called directly; passed to an
+ * fft_apply_kernel* function, their address selects the device kernel.  Any
+ * other callback returns MITOS_ERR_UNSUPPORTED (no host fallback). */
+void kernel_constant(struct kernel *the_kernel);          /* params: const double* */
+void kernel_inv_poisson(struct kernel *the_kernel);       /* params: unused        */
+void kernel_dx(struct kernel *the_kernel);
+void kernel_dy(struct kernel *the_kernel);
+void kernel_dz(struct kernel *the_kernel);
+void kernel_power_no_transfer(struct kernel *the_kernel); /* params: struct cosmology* */
+void kernel_undo_Hermite_window(struct kernel *the_kernel); /* params: Hermite_kern_params* */
+/* closed-form spectra (new): params point to struct mitos_spectrum_params */
+struct mitos_spectrum_params {
+    double A;     /* amplitude of P(k) = A k^n_s T(k)^2 */
+    double n_s;
+    double Gamma; /* BBKS shape parameter (k in h/Mpc) */
+};
+void kernel_sqrt_power_law(struct kernel *the_kernel); /* sqrt(A k^n_s), 0 at k=0 */
+void kernel_bbks_transfer(struct kernel *the_kernel);  /* T_BBKS(k/Gamma), 0 at k=0 */
+
+/* include/poisson.h:29-30 */
+int solvePoisson(double *phi, double *f, int N, double boxlen); /* device N^3 arrays */
+int solvePoisson_dg(struct distributed_grid *dg);
+
+/* include/2lpt.h:27-29 */
+int solve2LPT(struct distributed_grid *potential, struct distributed_grid *density,
+              struct distributed_grid *workspace, double factor1, double factor2);
+
+/* include/calc_powerspec.h:26-36 -- box1/box2 device, bin arrays host.  The
+ * reference returns void; a failure leaves NaN bins and mitos_last_error() set */
+void calc_cross_powerspec(int N, double boxlen, const fftw_complex *box1,
+                          const fftw_complex *box2, int bins, double *k_in_bins,
+                          double *power_in_bins, int *obs_in_bins);
+void calc_cross_powerspec_float(int N, double boxlen, const fftwf_complex *box1,
+                                const fftwf_complex *box2, int bins, double *k_in_bins,
+                                double *power_in_bins, int *obs_in_bins);
+void calc_cross_powerspec_float_mpi(long int N, long int X0, long int NX, double boxlen,
+                                    const fftwf_complex *box1, const fftwf_complex *box2,
+                                    int bins, double *k_in_bins, double *power_in_bins,
+                                    long int *obs_in_bins);
+
+/* include/grids_interp.h:41-48 -- device arrays */
+int undoCICWindow(fftw_complex *farr, int N, double boxlen);
+int undoTSCWindow(fftw_complex *farr, int N, double boxlen);
+int undoCICWindowFloat(fftwf_complex *farr, int N, double boxlen);
+int undoTSCWindowFloat(fftwf_complex *farr, int N, double boxlen);
+
+/* The FFTW plans the reference builds inline (src/poisson.c:35-36,
+ * analyse/src/mitos_read.c:411-413, analyse/src/mitos_dist_read.c:413):
+ * unnormalised 3D transforms of unpadded N^3 device arrays. */
+int mitos_fft_r2c(int N, const double *in, fftw_complex *out);
+int mitos_fft_c2r(int N, const fftw_complex *in, double *out);
+int mitos_fft_r2c_float(int N, const float *in, fftwf_complex *out);
+int mitos_fft_c2r_float(int N, const fftwf_complex *in, float *out);
+
+/* host-side table of the reference P(k) bin for every integer
+ * n^2 = nx^2+ny^2+nz^2 in [0, 3(N/2)^2] (-1 for n^2 = 0); table has
+ * 3(N/2)^2+1 entries.  No device work. */
+int mitos_pk_bin_table(int N, double boxlen, int bins, int *table);
+
+/* last error of a void reference function; mitos_strerror for any code */
+int mitos_last_error(void);
+const char *mitos_strerror(int code);
+/* free the library's cached device scratch buffers */
+int mitos_release_workspace(void);
+
+/* ======================================================================== */
+/* Part 2: batched particle steps (replace per-particle reference loops)     */
+/* ======================================================================== */
+
+/* src/particle.c:87-129 with FirstID (src/particle_types.c:102-104): M^3
+ * lattice particles, X = x*((float)L/M) in float, id = x*M*M+y*M+z+first_id.
+ * Device arrays; ids may be NULL. */
+int mitos_lattice(int M, double boxlen, long long first_id, float *X, float *Y, float *Z,
+                  long long *ids);
+
+/* src/mitos.c:827-863 then :1072-1077: for dir = x, y, z in turn, subtract the
+ * TSC-interpolated displacement grid (gridTSC_dg, src/grids_interp.c:267-309)
+ * at the particle's current float position, then wrap into [0, L).  The
+ * grids are in configuration space (padded box arrays). */
+int mitos_displace_tsc_dg(const struct distributed_grid *psi_x,
+                          const struct distributed_grid *psi_y,
+                          const struct distributed_grid *psi_z, long long npart, float *X,
+                          float *Y, float *Z);
+
+/* analyse/src/mitos_read.c:278-376: CIC mass assignment of npart particles of
+ * equal mass into an unpadded N^3 device grid, then (box-avg)/avg.  The
+ * deposit accumulates 2^-40-unit fixed point, so the result does not depend
+ * on thread order. */
+int mitos_cic_overdensity(int N, double boxlen, long long npart, const float *X,
+                          const float *Y, const float *Z, double *box);
+
+/* ======================================================================== */
+/* Part 3: fused pipeline (GRF -> 2LPT -> displacement -> CIC -> P(k))      */
+/* ======================================================================== */
+
+#define MITOS_FP64 0
+#define MITOS_FP32 1
+
+struct mitos_ic_config {
+    int N;          /* [Box] GridSize; particle lattice M = N */
+    double boxlen;  /* [Box] BoxLen (Mpc/h) */
+    uint64_t seed;  /* [Random] Seed (stream of rank 0, src/mitos.c:148) */
+    double n_s;     /* [Cosmology] n_s */
+    double sigma8;  /* [Spectrum] Sigma8 (top hat R = 8 Mpc/h) */
+    double gamma;   /* [Spectrum] Gamma (BBKS shape) */
+    double f1, f2;  /* 2LPT factors, (1, -3/7) as src/mitos.c:385 */
+    int run2lpt;    /* [ParticleType_0] Run2LPT (0: Zel'dovich) */
+    int bins;       /* [Read] PowerSpectrumBins */
+    int precision;  /* MITOS_FP64 or MITOS_FP32 */
+};
+
+struct mitos_ic_result {
+    /* host arrays of cfg.bins entries, caller-owned (may be NULL) */
+    double *k_in_bins;
+    double *power_in_bins;
+    long long *obs_in_bins;
+    double amplitude;   /* A of P(k) = A k^n_s T(k)^2 */
+    float stage_ms[8];  /* noise, hessian, poisson, displacement, particles, cic, pk, total */
+};
+
+typedef struct mitos_ic_context mitos_ic_context;
+
+/* sigma8 normalisation of P(k) = A k^n_s T_BBKS(k)^2 (host) */
+double mitos_sigma8_amplitude(double n_s, double gamma, double sigma8, double R);
+
+/* read the .ini keys of the path (see struct mitos_ic_config) */
+int mitos_read_ini(const char *fname, struct mitos_ic_config *cfg);
+
+int mitos_ic_create(const struct mitos_ic_config *cfg, mitos_ic_context **ctx);
+/* run one realization; cfg must keep N and precision of the context */
+int mitos_ic_run(mitos_ic_context *ctx, const struct mitos_ic_config *cfg,
+                 struct mitos_ic_result *res);
+/* the same, issued on `stream` without a host synchronisation */
+int mitos_ic_enqueue(mitos_ic_context *ctx, const struct mitos_ic_config *cfg, void *stream);
+int mitos_ic_fetch_pk(mitos_ic_context *ctx, struct mitos_ic_result *res);
+/* copy the last run's particle positions (N^3 floats each) into caller
+ * device arrays */
+int mitos_ic_positions(mitos_ic_context *ctx, float *X, float *Y, float *Z);
+/* copy the last run's overdensity (unpadded N^3) or its three displacement
+ * grids (3 x N^3, x then y then z, unpadded) into a caller device buffer of
+ * the context's real type (float for MITOS_FP32, double for MITOS_FP64) */
+int mitos_ic_density(mitos_ic_context *ctx, void *delta_device);
+int mitos_ic_displacements(mitos_ic_context *ctx, void *psi_device);
+/* particle ids (device, N^3) written on demand, closed form of the lattice */
+int mitos_ic_ids(mitos_ic_context *ctx, long long *ids_device);
+int mitos_ic_destroy(mitos_ic_context *ctx);
+
+/* the reference's call order through the Part 1 functions (FP64 plain path) */
+int mitos_ic_run_reference_order(const struct mitos_ic_config *cfg,
+                                 struct mitos_ic_result *res, float *X, float *Y, float *Z);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MITOS_HIP_H */

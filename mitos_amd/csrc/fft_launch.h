@@ -1,0 +1,197 @@
+// Host-side launch layer for the FFT passes: size dispatch, twiddle tables,
+// one launcher per pass kind and the composed 3D transforms.
+#pragma once
+
+#include <map>
+#include <mutex>
+
+#include "fft_passes.h"
+
+namespace mitos {
+
+// Grid sizes with compiled FFT passes (line length N for the x/y passes, N/2
+// for the z passes).
+template <typename F>
+int with_size(int N, F&& f) {
+    switch (N) {
+        case 8: return f.template operator()<8>();
+        case 16: return f.template operator()<16>();
+        case 32: return f.template operator()<32>();
+        case 64: return f.template operator()<64>();
+        case 128: return f.template operator()<128>();
+        case 256: return f.template operator()<256>();
+        case 512: return f.template operator()<512>();
+        case 1024: return f.template operator()<1024>();
+        case 1536: return f.template operator()<1536>();
+        default: return MITOS_ERR_UNSUPPORTED;
+    }
+}
+
+inline bool size_supported(int N) {
+    return N == 8 || N == 16 || N == 32 || N == 64 || N == 128 || N == 256 || N == 512 ||
+           N == 1024 || N == 1536;
+}
+
+// Device twiddle table tw[k] = exp(-2 pi i k / L), k < L, one per (device, L, type).
+template <typename T>
+const cx<T>* twiddles(int L) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, cx<T>*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_pair(dev, L);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    cx<T>* h = new cx<T>[L];
+    for (int k = 0; k < L; k++) {
+        // reduce by symmetry so the quarter-period values are exact
+        long n = k, d = L;
+        double c, s;
+        if (4 * n == d) { c = 0; s = 1; }
+        else if (2 * n == d) { c = -1; s = 0; }
+        else if (4 * n == 3 * d) { c = 0; s = -1; }
+        else if (n == 0) { c = 1; s = 0; }
+        else {
+            const double a = 2.0 * M_PI * (double)n / (double)d;
+            c = cos(a);
+            s = sin(a);
+        }
+        h[k] = mk<T>((T)c, (T)(-s));
+    }
+    cx<T>* d = nullptr;
+    if (hipMalloc(&d, sizeof(cx<T>) * L) != hipSuccess) {
+        delete[] h;
+        return nullptr;
+    }
+    if (hipMemcpy(d, h, sizeof(cx<T>) * L, hipMemcpyHostToDevice) != hipSuccess) {
+        delete[] h;
+        hipFree(d);
+        return nullptr;
+    }
+    delete[] h;
+    cache[key] = d;
+    return d;
+}
+
+inline int launch_status() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "mitos_hip: kernel launch failed: %s\n", hipGetErrorString(e));
+        return MITOS_ERR_DEVICE;
+    }
+    return MITOS_OK;
+}
+
+// ---- strided pass ----------------------------------------------------------
+template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, typename RAD>
+struct StridedK;
+template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, int... Rs>
+struct StridedK<T, L, C, AXIS, DIR, SPLIT, OP, Radices<Rs...>> {
+    static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
+                   int ncols, long bstride, const cx<T>* tw, OP op) {
+        hipLaunchKernelGGL(
+            (k_pass_strided<T, L, LineCfg<L>::E, C, AXIS, DIR, SPLIT, OP, Rs...>), g, b, 0, s,
+            in, out, hs, ncols, bstride, tw, op);
+    }
+};
+
+template <typename T, int L, int AXIS, int DIR, typename OP>
+int strided_pass(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride, OP op,
+                 hipStream_t s) {
+    constexpr int C = ColTile<T>::C;
+    constexpr int E = LineCfg<L>::E;
+    constexpr int TPL = L / E;
+    constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 65536;
+    const int ncols = L / 2 + 1;
+    const cx<T>* tw = twiddles<T>(L);
+    if (!tw) return MITOS_ERR_ALLOC;
+    dim3 grid((ncols + C - 1) / C, L, nbatch);
+    StridedK<T, L, C, AXIS, DIR, SPLIT, OP, typename LineCfg<L>::R>::go(
+        grid, dim3(TPL * C), s, in, out, hs, ncols, bstride, tw, op);
+    return launch_status();
+}
+
+// ---- z passes ----------------------------------------------------------------
+template <int TPL>
+struct RowsPerBlock {
+    static constexpr int value = TPL >= 256 ? 1 : 256 / TPL;
+};
+
+template <typename T, int L, int RW, typename OP, typename RAD>
+struct ZC2RK;
+template <typename T, int L, int RW, typename OP, int... Rs>
+struct ZC2RK<T, L, RW, OP, Radices<Rs...>> {
+    template <typename... A>
+    static void go(dim3 g, dim3 b, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_zc2r<T, L, LineCfg<L>::E, RW, OP, Rs...>), g, b, 0, s, a...);
+    }
+};
+
+template <typename T, int N, typename OP>
+int zc2r_pass(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, long outb,
+              T scale, OP op, hipStream_t s) {
+    constexpr int L = N / 2;
+    constexpr int TPL = L / LineCfg<L>::E;
+    constexpr int RW = RowsPerBlock<TPL>::value;
+    const long nrows = (long)N * N;
+    const cx<T>* tw = twiddles<T>(L);
+    const cx<T>* twN = twiddles<T>(N);
+    if (!tw || !twN) return MITOS_ERR_ALLOC;
+    dim3 grid((unsigned)((nrows + RW - 1) / RW), nbatch);
+    ZC2RK<T, L, RW, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, in, out, nrows,
+                                                     N, hs, rs, inb, outb, tw, twN, scale, op);
+    return launch_status();
+}
+
+template <typename T, int L, int RW, typename LOADER, typename OP, typename RAD>
+struct ZR2CK;
+template <typename T, int L, int RW, typename LOADER, typename OP, int... Rs>
+struct ZR2CK<T, L, RW, LOADER, OP, Radices<Rs...>> {
+    template <typename... A>
+    static void go(dim3 g, dim3 b, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_zr2c<T, L, LineCfg<L>::E, RW, LOADER, OP, Rs...>), g, b, 0, s,
+                           a...);
+    }
+};
+
+template <typename T, int N, typename LOADER, typename OP>
+int zr2c_pass(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op, hipStream_t s) {
+    constexpr int L = N / 2;
+    constexpr int TPL = L / LineCfg<L>::E;
+    constexpr int RW = RowsPerBlock<TPL>::value;
+    const long nrows = (long)N * N;
+    const cx<T>* tw = twiddles<T>(L);
+    const cx<T>* twN = twiddles<T>(N);
+    if (!tw || !twN) return MITOS_ERR_ALLOC;
+    dim3 grid((unsigned)((nrows + RW - 1) / RW), nbatch);
+    ZR2CK<T, L, RW, LOADER, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, loader,
+                                                             out, nrows, N, hs, outb, tw, twN,
+                                                             op);
+    return launch_status();
+}
+
+// ---- composed 3D transforms ---------------------------------------------------
+// c2r: x (in -> ws), y (ws -> ws), z (ws -> out).  The input is left unchanged.
+template <typename T, typename OPX, typename OPY, typename OPZ>
+int c2r_3d(const cx<T>* in, cx<T>* ws, T* out, const GridLayout& g, int nb, long cb, long rb,
+           T scale, OPX ox, OPY oy, OPZ oz, hipStream_t s) {
+    return with_size(g.N, [&]<int NN>() -> int {
+        MITOS_TRY((strided_pass<T, NN, 0, 1>(in, ws, g.hs, nb, cb, ox, s)));
+        MITOS_TRY((strided_pass<T, NN, 1, 1>(ws, ws, g.hs, nb, cb, oy, s)));
+        return zc2r_pass<T, NN>(ws, out, g.hs, g.rs, nb, cb, rb, scale, oz, s);
+    });
+}
+
+// r2c: z (loader -> out), y (out -> out), x (out -> out).
+template <typename T, typename LOADER, typename OPZ, typename OPY, typename OPX>
+int r2c_3d(LOADER loader, cx<T>* out, const GridLayout& g, int nb, long cb, OPZ oz, OPY oy,
+           OPX ox, hipStream_t s) {
+    return with_size(g.N, [&]<int NN>() -> int {
+        MITOS_TRY((zr2c_pass<T, NN>(loader, out, g.hs, nb, cb, oz, s)));
+        MITOS_TRY((strided_pass<T, NN, 1, -1>(out, out, g.hs, nb, cb, oy, s)));
+        return strided_pass<T, NN, 0, -1>(out, out, g.hs, nb, cb, ox, s);
+    });
+}
+
+}  // namespace mitos

@@ -1,0 +1,226 @@
+// 3D real<->complex FFT passes over the half-complex grid layout.
+//
+//  strided pass : c2c lines along x (stride N*hs) or y (stride hs).  A
+//                 work-group takes C consecutive z' columns at one value of the
+//                 other coordinate, so every load/store instruction of a wave
+//                 covers rows of C contiguous complex values.
+//  z c2r pass   : one row of N/2+1 complex values -> N reals, through an
+//                 N/2-point complex FFT (even/odd packing).  The imaginary
+//                 parts of the z'=0 and z'=N/2 entries are ignored, which is
+//                 how a c2r treats a non-Hermitian last-axis spectrum.
+//  z r2c pass   : N reals -> N/2+1 complex values, same packing.
+//
+// Each pass takes an operator object that may rewrite a value on load (pre)
+// or before the store (post) given its grid coordinates, so k-space kernels
+// and normalisations fuse into the passes.
+#pragma once
+
+#include "fft_core.h"
+
+namespace mitos {
+
+template <int... Rs>
+struct Radices {};
+
+// line configuration per length: E values per thread and the stage radices
+template <int L>
+struct LineCfg;
+template <> struct LineCfg<4>    { static constexpr int E = 4;  typedef Radices<4> R; };
+template <> struct LineCfg<8>    { static constexpr int E = 8;  typedef Radices<8> R; };
+template <> struct LineCfg<16>   { static constexpr int E = 16; typedef Radices<16> R; };
+template <> struct LineCfg<32>   { static constexpr int E = 8;  typedef Radices<8, 4> R; };
+template <> struct LineCfg<64>   { static constexpr int E = 8;  typedef Radices<8, 8> R; };
+template <> struct LineCfg<128>  { static constexpr int E = 16; typedef Radices<16, 8> R; };
+template <> struct LineCfg<256>  { static constexpr int E = 16; typedef Radices<16, 16> R; };
+template <> struct LineCfg<512>  { static constexpr int E = 8;  typedef Radices<8, 8, 8> R; };
+template <> struct LineCfg<768>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 4, 3> R; };
+template <> struct LineCfg<1024> { static constexpr int E = 16; typedef Radices<16, 16, 4> R; };
+template <> struct LineCfg<1536> { static constexpr int E = 24; typedef Radices<8, 8, 8, 3> R; };
+
+// columns per strided tile: 128-byte rows (16 float2 / 8 double2)
+template <typename T>
+struct ColTile {
+    static constexpr int C = 128 / (2 * sizeof(T));
+};
+
+// ---- operators ------------------------------------------------------------
+// pre/post(v, a, b, c): (a, b, c) are the (x, y, z') grid coordinates of the
+// value in half-complex index space.
+struct OpNone {
+    template <typename T>
+    __device__ __forceinline__ cx<T> pre(cx<T> v, int, int, int) const { return v; }
+    template <typename T>
+    __device__ __forceinline__ cx<T> post(cx<T> v, int, int, int) const { return v; }
+};
+
+template <typename T>
+struct OpScale {
+    T s;
+    __device__ __forceinline__ cx<T> pre(cx<T> v, int, int, int) const { return v; }
+    __device__ __forceinline__ cx<T> post(cx<T> v, int, int, int) const { return cscale(v, s); }
+};
+
+// ---- strided c2c pass ------------------------------------------------------
+// AXIS 0: lines along x, blockIdx.y = y;  AXIS 1: lines along y, blockIdx.y = x.
+// blockIdx.z selects one of several grids spaced `batch_stride` apart.
+// in == out is allowed: every thread stores exactly the elements it loaded.
+template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, typename OP,
+          int... Rs>
+__global__ void __launch_bounds__(L / E * C)
+    k_pass_strided(const cx<T>* in, cx<T>* out, long hs,
+                   int ncols, long batch_stride, const cx<T>* __restrict__ tw, OP op) {
+    constexpr int TPL = L / E;
+    typedef LdsMap<T, L, C, true> M;
+    __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
+    const int c = threadIdx.x % C;
+    const int t = threadIdx.x / C;
+    const int col = blockIdx.x * C + c;
+    const int o = blockIdx.y;
+    const bool ok = col < ncols;
+    const long pstride = (AXIS == 0) ? (long)L * hs : hs;
+    const long ostride = (AXIS == 0) ? hs : (long)L * hs;
+    const long base = blockIdx.z * batch_stride + (long)o * ostride + col;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int p = t + TPL * e;
+        cx<T> a = ok ? ld(in + base + (long)p * pstride) : mk<T>(0, 0);
+        v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
+    }
+    line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int p = t + TPL * e;
+            cx<T> a = (AXIS == 0) ? op.post(v[e], p, o, col) : op.post(v[e], o, p, col);
+            st(out + base + (long)p * pstride, a);
+        }
+    }
+}
+
+// ---- z c2r pass ------------------------------------------------------------
+// in: rows of hs complex (first N/2+1 used); out: rows of rs reals (first N
+// written, the remaining rs-N set to zero).  L = N/2.
+template <typename T, int L, int E, int RW, typename OP, int... Rs>
+__global__ void __launch_bounds__(L / E * RW)
+    k_zc2r(const cx<T>* __restrict__ in, T* __restrict__ out, long nrows, int N, long hs,
+           long rs, long in_batch, long out_batch, const cx<T>* __restrict__ tw,
+           const cx<T>* __restrict__ twN, T scale, OP op) {
+    constexpr int TPL = L / E;
+    typedef LdsMap<T, L, RW, false> M;
+    __shared__ T lre[M::SIZE];
+    __shared__ T lim[M::SIZE];
+    const int t = threadIdx.x % TPL;
+    const int rl = threadIdx.x / TPL;
+    const long row = (long)blockIdx.x * RW + rl;
+    const bool ok = row < nrows;
+    const cx<T>* src = in + blockIdx.y * in_batch + row * hs;
+    const int gx = (int)(row / N), gy = (int)(row % N);
+    // stage the row (positions 0..L) in LDS
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = t + TPL * e;
+            cx<T> a = op.pre(ld(src + k), gx, gy, k);
+            lre[M::idx(k, rl)] = a.re;
+            lim[M::idx(k, rl)] = a.im;
+        }
+        if (t == 0) {
+            cx<T> a = op.pre(ld(src + L), gx, gy, L);
+            lre[M::idx(L, rl)] = a.re;
+            lim[M::idx(L, rl)] = a.im;
+        }
+    }
+    __syncthreads();
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = t + TPL * e;
+        cx<T> xk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+        cx<T> xm = mk<T>(lre[M::idx(L - k, rl)], lim[M::idx(L - k, rl)]);
+        if (k == 0) {  // Im of the DC and Nyquist entries is ignored
+            xk.im = 0;
+            xm.im = 0;
+        }
+        cx<T> a = xk + conjg(xm);
+        cx<T> b = xk - conjg(xm);
+        cx<T> w = twN[k];
+        w.im = -w.im;  // exp(+2 pi i k / N)
+        v[e] = a + mul_i<1>(cmul(w, b));
+    }
+    __syncthreads();
+    line_fft<T, L, E, RW, false, 1, Rs...>(v, lre, lim, tw, t, rl);
+    if (ok) {
+        T* dst = out + blockIdx.y * out_batch + row * rs;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int m = t + TPL * e;
+            typename vec2<T>::type p;
+            p.x = v[e].re * scale;
+            p.y = v[e].im * scale;
+            *reinterpret_cast<typename vec2<T>::type*>(dst + 2 * m) = p;
+        }
+        for (long z = 2 * L + t; z < rs; z += TPL) dst[z] = 0;
+    }
+}
+
+// ---- z r2c pass ------------------------------------------------------------
+// LOADER: functor returning the pair (x[2m], x[2m+1]) of row `row` as cx<T>.
+template <typename T>
+struct RealRowLoader {
+    const T* __restrict__ in;
+    long rs;
+    long batch;
+    __device__ __forceinline__ cx<T> operator()(long b, long row, int m) const {
+        typename vec2<T>::type p =
+            *reinterpret_cast<const typename vec2<T>::type*>(in + b * batch + row * rs + 2 * m);
+        return mk<T>(p.x, p.y);
+    }
+};
+
+template <typename T, int L, int E, int RW, typename LOADER, typename OP, int... Rs>
+__global__ void __launch_bounds__(L / E * RW)
+    k_zr2c(LOADER loader, cx<T>* __restrict__ out, long nrows, int N, long hs, long out_batch,
+           const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN, OP op) {
+    constexpr int TPL = L / E;
+    typedef LdsMap<T, L, RW, false> M;
+    __shared__ T lre[M::SIZE];
+    __shared__ T lim[M::SIZE];
+    const int t = threadIdx.x % TPL;
+    const int rl = threadIdx.x / TPL;
+    const long row = (long)blockIdx.x * RW + rl;
+    const bool ok = row < nrows;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = ok ? loader(blockIdx.y, row, t + TPL * e) : mk<T>(0, 0);
+    line_fft<T, L, E, RW, false, -1, Rs...>(v, lre, lim, tw, t, rl);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int m = t + TPL * e;
+        lre[M::idx(m, rl)] = v[e].re;
+        lim[M::idx(m, rl)] = v[e].im;
+    }
+    __syncthreads();
+    if (!ok) return;
+    const int gx = (int)(row / N), gy = (int)(row % N);
+    cx<T>* dst = out + blockIdx.y * out_batch + row * hs;
+    const T h = (T)0.5;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = t + TPL * e;
+        const int km = (k == 0) ? 0 : L - k;
+        cx<T> zk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+        cx<T> zm = conjg(mk<T>(lre[M::idx(km, rl)], lim[M::idx(km, rl)]));
+        cx<T> ev = cscale(zk + zm, h);
+        cx<T> od = cscale(zk - zm, h);                 // = i * O[k]
+        cx<T> w = twN[k];                              // exp(-2 pi i k / N)
+        cx<T> xk = ev + mul_i<-1>(cmul(w, od));        // E + w * O
+        st(dst + k, op.post(xk, gx, gy, k));
+    }
+    if (t == 0) {
+        cx<T> z0 = mk<T>(lre[M::idx(0, rl)], lim[M::idx(0, rl)]);
+        st(dst + L, op.post(mk<T>(z0.re - z0.im, (T)0), gx, gy, L));
+    }
+}
+
+}  // namespace mitos

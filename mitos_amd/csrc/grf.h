@@ -1,0 +1,131 @@
+// Gaussian random field: white noise from the xoshiro256** stream in the
+// reference's draw order, with a k-dependent amplitude fused in.
+#pragma once
+
+#include "crmath.h"
+#include "rng.h"
+
+namespace mitos {
+
+// Per-grid-size jump data, resident on the device.
+struct NoiseJumps {
+    const uint64_t* tab;  // nbits matrices J^(2^b), J = T^(4*(N/2+1)), 256*4 words each
+    int nbits;
+    Xo s0;    // state before the first draw (row 0 starts here)
+    Xo base;  // state after 4*(N/2+1) - 4 draws (row 1 starts here)
+};
+
+// Host: jump data for grid size N starting from state s0 (table cached per N).
+int noise_jumps(int N, const Xo& s0, NoiseJumps* out);
+// Host: state after the whole grid's draws, 4*(N*N*(N/2+1) - 1).
+int noise_advance(int N, Xo* st);
+
+// start state of z-row `line` = x*N + y.  Row 0 begins with the DC mode,
+// which takes no draw (src/grf.c:53-60); every other mode takes four.
+__device__ __forceinline__ Xo row_start(const NoiseJumps& J, long line) {
+    if (line == 0) return J.s0;
+    Xo s = J.base;
+    const unsigned long n = (unsigned long)(line - 1);
+    for (int b = 0; b < J.nbits; b++)
+        if ((n >> b) & 1UL) s = bm_apply_rows(J.tab + (size_t)b * 1024, s);
+    return s;
+}
+
+// src/random.c:34-49: cosine branch of Box-Muller from two 64-bit draws
+__device__ __forceinline__ double bm_normal(Xo& gen) {
+    const uint64_t A = xo_next(gen);
+    const uint64_t B = xo_next(gen);
+    const double RMax = 18446744073709551616.0;  // (double)UINT64_MAX + 1 = 2^64
+    const double u = ((double)A + 0.5) / RMax;
+    const double v = ((double)B + 0.5) / RMax;
+    return sqrt(-2 * cr_log(u)) * cr_cos(2 * M_PI * v);
+}
+
+// src/fft.c:29-35, evaluated without contraction so k matches the host bits
+__host__ __device__ __forceinline__ void wavevec(int x, int y, int z, int N, double dk,
+                                                 double& kx, double& ky, double& kz,
+                                                 double& k) {
+#pragma clang fp contract(off)
+    kx = (x > N / 2) ? (x - N) * dk : x * dk;
+    ky = (y > N / 2) ? (y - N) * dk : y * dk;
+    kz = (z > N / 2) ? (z - N) * dk : z * dk;
+    k = sqrt(kx * kx + ky * ky + kz * kz);
+}
+
+// src/grf.c:26-69 with an amplitude amp(kx,ky,kz,k) applied to both parts.
+// One thread generates one z-row in stream order.
+template <typename T, typename AMP>
+__global__ void __launch_bounds__(256)
+    k_grf(cx<T>* out, long hs, int N, double boxlen, long batch_stride, NoiseJumps J,
+          AMP amp) {
+    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= (long)N * N) return;
+    const int x = (int)(line / N), y = (int)(line % N);
+    Xo gen = row_start(J, line);
+    const double boxvol = boxlen * boxlen * boxlen;
+    const double factor = sqrt(boxvol / 2);
+    const double dk = 2 * M_PI / boxlen;
+    cx<T>* row = out + blockIdx.y * batch_stride + line * hs;
+    for (int z = 0; z <= N / 2; z++) {
+        double kx, ky, kz, k;
+        wavevec(x, y, z, N, dk, kx, ky, kz, k);
+        if (k > 0) {
+            const double a = bm_normal(gen) * factor;
+            const double b = bm_normal(gen) * factor;
+            const double m = amp(kx, ky, kz, k);
+            st(row + z, mk<T>((T)(a * m), (T)(b * m)));
+        } else {
+            st(row + z, mk<T>(0, 0));
+        }
+    }
+}
+
+// src/grf.c:78-175 on one rank.  In place: every lower-half point reads its
+// mirror, which lies in the untouched upper half or is the point itself.
+template <typename T>
+__global__ void k_hermitian(cx<T>* f, long hs, int N, long batch_stride) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)(N / 2 + 1) * N;
+    if (i >= 2 * per) return;
+    const int z = (i < per) ? 0 : N / 2;
+    const long j = i % per;
+    const int x = (int)(j / N), y = (int)(j % N);
+    if ((x == 0 || x == N / 2) && y > N / 2) return;
+    const int invx = (x > 0) ? N - x : 0;
+    const int invy = (y > 0) ? N - y : 0;
+    const int invz = (z > 0) ? N - z : 0;
+    cx<T>* g = f + blockIdx.y * batch_stride;
+    const long id = ((long)x * N + y) * hs + z;
+    if (invx == x && invy == y && invz == z) {
+        g[id].im = 0;
+    } else {
+        const long mid = ((long)invx * N + invy) * hs + z;
+        g[id] = conjg(g[mid]);
+    }
+}
+
+struct AmpOne {
+    __device__ __forceinline__ double operator()(double, double, double, double) const {
+        return 1.0;
+    }
+};
+
+template <typename T, typename AMP>
+int launch_grf(cx<T>* out, long hs, int N, double boxlen, int nb, long bstride,
+               const NoiseJumps& J, AMP amp, hipStream_t s) {
+    const long rows = (long)N * N;
+    dim3 grid((unsigned)((rows + 255) / 256), nb);
+    hipLaunchKernelGGL((k_grf<T, AMP>), grid, dim3(256), 0, s, out, hs, N, boxlen, bstride, J,
+                       amp);
+    return hipGetLastError() == hipSuccess ? MITOS_OK : MITOS_ERR_DEVICE;
+}
+
+template <typename T>
+int launch_hermitian(cx<T>* f, long hs, int N, int nb, long bstride, hipStream_t s) {
+    const long n = 2L * (N / 2 + 1) * N;
+    dim3 grid((unsigned)((n + 255) / 256), nb);
+    hipLaunchKernelGGL((k_hermitian<T>), grid, dim3(256), 0, s, f, hs, N, bstride);
+    return hipGetLastError() == hipSuccess ? MITOS_OK : MITOS_ERR_DEVICE;
+}
+
+}  // namespace mitos

@@ -1,0 +1,309 @@
+// Fused kernels of the initial-condition pipeline.
+//
+// Per realization the pipeline keeps every field in HBM and replaces the
+// reference's 19 separate 3D FFTs plus elementwise passes by 11 transforms'
+// worth of fused passes:
+//   noise+spectrum -> phi1(k) = -delta(k)/k^2
+//   x-pass (1 in, 3 out), y-pass (3 in, 6 out): the six Hessian components in
+//     (x, y, k_z) space, factored so each FFT line is read once
+//   z-pass: six c2r rows -> delta2 = sum of Hessian minors -> r2c row
+//   y-pass forward, x-pass forward + Poisson + 2LPT combine + TSC window +
+//     first inverse pass of the displacement fields (1 in + phi1, 2 out)
+//   y-pass (2 in, 3 out), z c2r x3 -> displacement grids
+//   TSC displacement of the lattice, fixed-point CIC, r2c with the integer
+//     grid converted on load, P(k) with the CIC window and normalisation.
+#pragma once
+
+#include "fft_passes.h"
+#include "kspace.h"
+#include "particles.h"
+
+namespace mitos {
+
+// ---- multi-output strided pass ------------------------------------------------
+// Output j = FFT_DIR( m_j(k_axis) * in[src_j] ) where the multiplier code is
+// 0: 1, 1: i k, 2: -k^2 (k = signed frequency of the line position times dk).
+template <typename T, int NIN, int NOUT>
+struct MultiPtrs {
+    const cx<T>* in[NIN];
+    cx<T>* out[NOUT];
+};
+
+template <int... CODES>
+struct OutCodes {
+    static constexpr int n = sizeof...(CODES);
+    static constexpr int code[sizeof...(CODES)] = {CODES...};
+    static constexpr int src(int j) { return code[j] >> 2; }
+    static constexpr int mul(int j) { return code[j] & 3; }
+};
+#define MITOS_OUT(src, mul) ((src) * 4 + (mul))
+
+template <typename T>
+__device__ __forceinline__ cx<T> apply_mul(int code, cx<T> v, T k) {
+    if (code == 1) return mk<T>(-v.im * k, v.re * k);  // i k v
+    if (code == 2) {
+        const T m = -(k * k);
+        return mk<T>(v.re * m, v.im * m);
+    }
+    return v;
+}
+
+template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, int NIN,
+          typename CODES, int... Rs>
+__global__ void __launch_bounds__(L / E * C)
+    k_multi_strided(MultiPtrs<T, NIN, CODES::n> P, long hs, int ncols, double dk,
+                    const cx<T>* __restrict__ tw) {
+    constexpr int TPL = L / E;
+    constexpr int NOUT = CODES::n;
+    typedef LdsMap<T, L, C, true> M;
+    __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
+    const int c = threadIdx.x % C;
+    const int t = threadIdx.x / C;
+    const int col = blockIdx.x * C + c;
+    const int o = blockIdx.y;
+    const bool ok = col < ncols;
+    const long pstride = (AXIS == 0) ? (long)L * hs : hs;
+    const long ostride = (AXIS == 0) ? hs : (long)L * hs;
+    const long base = (long)o * ostride + col;
+    cx<T> v[E], u[E];
+    T kax[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) kax[e] = (T)(freq_index(t + TPL * e, L) * dk);
+#pragma unroll
+    for (int j = 0; j < NOUT; j++) {
+        if (j == 0 || CODES::src(j) != CODES::src(j > 0 ? j - 1 : 0)) {
+            const cx<T>* in = P.in[CODES::src(j)];
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                v[e] = ok ? ld(in + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) u[e] = apply_mul(CODES::mul(j), v[e], kax[e]);
+        line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+        if (ok) {
+            cx<T>* out = P.out[j];
+#pragma unroll
+            for (int e = 0; e < E; e++) st(out + base + (long)(t + TPL * e) * pstride, u[e]);
+        }
+    }
+}
+
+
+// ---- z pass: six Hessian rows -> delta2 -> r2c row ----------------------------
+// Inputs (rows of hs complex, (x, y, k_z) space): B[0] = Y(A0), B[1] = Y(i ky A0),
+// B[2] = Y(-ky^2 A0), B[3] = Y(A1), B[4] = Y(i ky A1), B[5] = Y(A2).
+// Hessian rows: xx = B5, yy = B2, zz = -kz^2 B0, xy = B4, yz = i kz B1,
+// xz = i kz B3.  delta2 follows src/2lpt.c:111-112 term by term.
+template <typename T, int L, int E, int RW, int... Rs>
+__global__ void __launch_bounds__(L / E * RW)
+    k_hess_z(MultiPtrs<T, 6, 1> P, long nrows, long hs, double dk, T scale,
+             const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN) {
+    constexpr int TPL = L / E;
+    typedef LdsMap<T, L, RW, false> M;
+    __shared__ T lre[M::SIZE];
+    __shared__ T lim[M::SIZE];
+    const int t = threadIdx.x % TPL;
+    const int rl = threadIdx.x / TPL;
+    const long row = (long)blockIdx.x * RW + rl;
+    const bool ok = row < nrows;
+    cx<T> xx[E], yy[E], s[E], cur[E];
+    // component order: xx, yy, zz, xy, yz, xz
+    const int srcs[6] = {5, 2, 0, 4, 1, 3};
+    const int muls[6] = {0, 0, 2, 0, 1, 1};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        // stage the row, with its k_z multiplier, in LDS
+        if (ok) {
+            const cx<T>* src = P.in[srcs[q]] + row * hs;
+#pragma unroll
+            for (int e = 0; e <= E; e++) {
+                const int k = (e < E) ? t + TPL * e : L;
+                if (e == E && t != 0) break;
+                cx<T> a = apply_mul(muls[q], ld(src + k), (T)(k * dk));
+                lre[M::idx(k, rl)] = a.re;
+                lim[M::idx(k, rl)] = a.im;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = t + TPL * e;
+            cx<T> xk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+            cx<T> xm = mk<T>(lre[M::idx(L - k, rl)], lim[M::idx(L - k, rl)]);
+            if (k == 0) {
+                xk.im = 0;
+                xm.im = 0;
+            }
+            cx<T> a = xk + conjg(xm);
+            cx<T> b = xk - conjg(xm);
+            cx<T> w = twN[k];
+            w.im = -w.im;
+            cur[e] = a + mul_i<1>(cmul(w, b));
+        }
+        __syncthreads();
+        line_fft<T, L, E, RW, false, 1, Rs...>(cur, lre, lim, tw, t, rl);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            // real values x[2m] = re, x[2m+1] = im
+            const cx<T> h = mk<T>(cur[e].re * scale, cur[e].im * scale);
+            if (q == 0) xx[e] = h;
+            else if (q == 1) yy[e] = h;
+            else if (q == 2) {
+                // d_dzz*d_dyy + d_dzz*d_dxx + d_dyy*d_dxx
+                s[e].re = h.re * yy[e].re + h.re * xx[e].re + yy[e].re * xx[e].re;
+                s[e].im = h.im * yy[e].im + h.im * xx[e].im + yy[e].im * xx[e].im;
+            } else {
+                // - d_dxy^2 - d_dyz^2 - d_dxz^2
+                s[e].re = s[e].re - h.re * h.re;
+                s[e].im = s[e].im - h.im * h.im;
+            }
+        }
+    }
+    // r2c of delta2 (packed even/odd) -> k_z row
+    line_fft<T, L, E, RW, false, -1, Rs...>(s, lre, lim, tw, t, rl);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int m = t + TPL * e;
+        lre[M::idx(m, rl)] = s[e].re;
+        lim[M::idx(m, rl)] = s[e].im;
+    }
+    __syncthreads();
+    if (!ok) return;
+    cx<T>* dst = P.out[0] + row * hs;
+    const T h = (T)0.5;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int k = t + TPL * e;
+        const int km = (k == 0) ? 0 : L - k;
+        cx<T> zk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+        cx<T> zm = conjg(mk<T>(lre[M::idx(km, rl)], lim[M::idx(km, rl)]));
+        cx<T> ev = cscale(zk + zm, h);
+        cx<T> od = cscale(zk - zm, h);
+        cx<T> xk = ev + mul_i<-1>(cmul(twN[k], od));
+        st(dst + k, xk);
+    }
+    if (t == 0) {
+        cx<T> z0 = mk<T>(lre[M::idx(0, rl)], lim[M::idx(0, rl)]);
+        st(dst + L, mk<T>(z0.re - z0.im, (T)0));
+    }
+}
+
+// ---- x pass: delta2 forward + Poisson + 2LPT combine + window + 2 inverse ----
+struct CombineParams {
+    double dk;
+    double sD;      // forward normalisation L^3/N^3 of delta2
+    double f1, f2;  // 2LPT factors
+    double wfac;    // 0.5 * boxlen / N, argument scale of the TSC window
+    int window;     // window order (3 = TSC), 0 = none
+};
+
+template <typename T, int L, int E, int C, bool SPLIT, int... Rs>
+__global__ void __launch_bounds__(L / E * C)
+    k_x_combine(const cx<T>* D, const cx<T>* __restrict__ K, cx<T>* C0, cx<T>* C1, long hs,
+                int ncols, CombineParams cp, const cx<T>* __restrict__ tw) {
+#pragma clang fp contract(off)
+    constexpr int TPL = L / E;
+    typedef LdsMap<T, L, C, true> M;
+    __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
+    const int c = threadIdx.x % C;
+    const int t = threadIdx.x / C;
+    const int col = blockIdx.x * C + c;
+    const int y = blockIdx.y;
+    const bool ok = col < ncols;
+    const long pstride = (long)L * hs;
+    const long base = (long)y * hs + col;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        v[e] = ok ? ld(D + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
+    line_fft<T, L, E, C, true, -1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    const double ky = freq_index(y, L) * cp.dk;
+    const double kz = col * cp.dk;
+    T kxs[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int x = t + TPL * e;
+        const double kx = freq_index(x, L) * cp.dk;
+        kxs[e] = (T)kx;
+        const double k = sqrt(kx * kx + ky * ky + kz * kz);
+        const cx<T> phi1 = ok ? ld(K + base + (long)x * pstride) : mk<T>(0, 0);
+        // phi2 = inv_poisson(delta2(k))  (include/fft_kernels.h:96-99)
+        const double ip = (k > 0) ? -1.0 / k / k : 1.0;
+        const double d2re = (double)v[e].re * cp.sD, d2im = (double)v[e].im * cp.sD;
+        double pre = (double)phi1.re * cp.f1 + cp.f2 * (d2re * ip);
+        double pim = (double)phi1.im * cp.f1 + cp.f2 * (d2im * ip);
+        if (cp.window && k > 0) {
+            // include/fft_kernels.h:198-219
+            const double W_x = sinc_d(kx * cp.wfac);
+            const double W_y = sinc_d(ky * cp.wfac);
+            const double W_z = sinc_d(kz * cp.wfac);
+            const double W = pow(W_x * W_y * W_z, (double)cp.window);
+            const double iw = 1.0 / W;
+            pre *= iw;
+            pim *= iw;
+        }
+        v[e] = mk<T>((T)pre, (T)pim);
+    }
+    cx<T> u[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) u[e] = apply_mul(1, v[e], kxs[e]);
+    line_fft<T, L, E, C, true, 1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < E; e++) st(C0 + base + (long)(t + TPL * e) * pstride, v[e]);
+    }
+    line_fft<T, L, E, C, true, 1, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < E; e++) st(C1 + base + (long)(t + TPL * e) * pstride, u[e]);
+    }
+}
+
+// ---- loaders / operators for the remaining passes -------------------------------
+// z c2r of the z displacement: multiply by i k_z on load
+struct OpIkz {
+    double dk;
+    template <typename T>
+    __device__ __forceinline__ cx<T> pre(cx<T> v, int, int, int z) const {
+        return apply_mul(1, v, (T)(z * dk));
+    }
+    template <typename T>
+    __device__ __forceinline__ cx<T> post(cx<T> v, int, int, int) const {
+        return v;
+    }
+};
+
+// density r2c: convert the fixed-point CIC grid to delta on load
+template <typename T>
+struct CicLoader {
+    const unsigned long long* __restrict__ grid;
+    long N;
+    double norm;  // 2^-40 * N^3 / npart
+    __device__ __forceinline__ cx<T> operator()(long, long row, int m) const {
+        const ulonglong2 q = *reinterpret_cast<const ulonglong2*>(grid + row * N + 2 * m);
+        return mk<T>((T)((double)q.x * norm - 1.0), (T)((double)q.y * norm - 1.0));
+    }
+};
+
+// P(k) value transform: forward normalisation and the CIC window (order 2)
+struct PkCicTransform {
+    double s;     // L^3/N^3
+    double wfac;  // 0.5 * boxlen / N
+    template <typename T>
+    __device__ __forceinline__ cx<double> operator()(cx<T> v, double kx, double ky, double kz,
+                                                     double k) const {
+#pragma clang fp contract(off)
+        double re = (double)v.re * s, im = (double)v.im * s;
+        if (k != 0) {
+            const double W_x = sinc_d(kx * wfac);
+            const double W_y = sinc_d(ky * wfac);
+            const double W_z = sinc_d(kz * wfac);
+            const double W = pow(W_x * W_y * W_z, 2.0);
+            re *= 1.0 / W;
+            im *= 1.0 / W;
+        }
+        return mk<double>(re, im);
+    }
+};
+
+}  // namespace mitos

@@ -1,0 +1,189 @@
+// Binned cross power spectrum (src/calc_powerspec.c:32-90) with a fixed,
+// thread-order-independent reduction: every wavefront folds equal-bin values
+// with a butterfly sum, lane 0 adds them into its own compensated (Neumaier)
+// accumulators in LDS, work-groups combine their wavefronts in order, and a
+// second kernel folds the work-group partials in order.
+#pragma once
+
+#include "fft_launch.h"
+#include "grf.h"
+#include "runtime.h"
+
+namespace mitos {
+
+__device__ __forceinline__ void kadd(double& s, double& c, double v) {
+#pragma clang fp contract(off)
+    const double t = s + v;
+    if (fabs(s) >= fabs(v))
+        c += (s - t) + v;
+    else
+        c += (v - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ long long wave_sum_i(long long v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct PkNoTransform {
+    template <typename T>
+    __device__ __forceinline__ cx<double> operator()(cx<T> v, double, double, double,
+                                                     double) const {
+        return mk<double>((double)v.re, (double)v.im);
+    }
+};
+
+// Partials layout: part[(blk*bins + b)*4 + {sumP, compP, sumK, compK}], cnt[blk*bins + b]
+template <typename T, typename XF>
+__global__ void __launch_bounds__(256)
+    k_pk_partial(const cx<T>* __restrict__ a, const cx<T>* __restrict__ b, int N, long hs,
+                 double dk, const int* __restrict__ lut, int bins, double* part,
+                 long long* cnt, XF xf) {
+#pragma clang fp contract(off)
+    extern __shared__ double sm[];  // [4 waves][bins][4] doubles + [4][bins] counts
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    double* acc = sm + (size_t)wv * bins * 4;
+    long long* cacc = reinterpret_cast<long long*>(sm + (size_t)4 * bins * 4) + (size_t)wv * bins;
+    for (int i = lane; i < bins * 4; i += 64) acc[i] = 0;
+    for (int i = lane; i < bins; i += 64) cacc[i] = 0;
+    __syncthreads();
+    const long H = N / 2 + 1;
+    const long nmodes = (long)N * N * H;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long base = (long)blockIdx.x * blockDim.x + wv * 64; base < nmodes; base += stride) {
+        const long i = base + lane;
+        int mybin = -1;
+        double vP = 0, vK = 0;
+        int mult = 0;
+        if (i < nmodes) {
+            const int z = (int)(i % H);
+            const long row = i / H;
+            const int y = (int)(row % N), x = (int)(row / N);
+            const int nx = freq_index(x, N), ny = freq_index(y, N);
+            const long n2 = (long)nx * nx + (long)ny * ny + (long)z * z;
+            if (n2 > 0) {
+                double kx, ky, kz, k;
+                wavevec(x, y, z, N, dk, kx, ky, kz, k);
+                const cx<double> u = xf(a[row * hs + z], kx, ky, kz, k);
+                const cx<double> v = xf(b[row * hs + z], kx, ky, kz, k);
+                mybin = lut[n2];
+                mult = (z == 0 || z == N / 2) ? 1 : 2;
+                vP = mult * (u.re * v.re + u.im * v.im);
+                vK = mult * k;
+            }
+        }
+        unsigned long long act = __ballot(mybin >= 0);
+        while (act) {
+            const int lead = __ffsll((long long)act) - 1;
+            const int bl = __shfl(mybin, lead, 64);
+            const bool mine = (mybin == bl);
+            const double sP = wave_sum(mine ? vP : 0.0);
+            const double sK = wave_sum(mine ? vK : 0.0);
+            const long long sC = wave_sum_i(mine ? (long long)mult : 0LL);
+            if (lane == 0) {
+                kadd(acc[bl * 4 + 0], acc[bl * 4 + 1], sP);
+                kadd(acc[bl * 4 + 2], acc[bl * 4 + 3], sK);
+                cacc[bl] += sC;
+            }
+            act &= ~__ballot(mine);
+        }
+    }
+    __syncthreads();
+    for (int bn = threadIdx.x; bn < bins; bn += blockDim.x) {
+        double sP = 0, cP = 0, sK = 0, cK = 0;
+        long long c = 0;
+        for (int w = 0; w < 4; w++) {
+            const double* aw = sm + (size_t)w * bins * 4 + bn * 4;
+            kadd(sP, cP, aw[0]);
+            kadd(sP, cP, aw[1]);
+            kadd(sK, cK, aw[2]);
+            kadd(sK, cK, aw[3]);
+            c += reinterpret_cast<long long*>(sm + (size_t)4 * bins * 4)[(size_t)w * bins + bn];
+        }
+        double* o = part + ((size_t)blockIdx.x * bins + bn) * 4;
+        o[0] = sP;
+        o[1] = cP;
+        o[2] = sK;
+        o[3] = cK;
+        cnt[(size_t)blockIdx.x * bins + bn] = c;
+    }
+}
+
+// fold the partials of nblk work-groups, in order; out[b*3 + {P_sum, K_sum}], obs[b]
+static __global__ void k_pk_final(const double* part, const long long* cnt, int nblk, int bins,
+                           double* out, long long* obs) {
+#pragma clang fp contract(off)
+    const int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn >= bins) return;
+    double sP = 0, cP = 0, sK = 0, cK = 0;
+    long long c = 0;
+    for (int g = 0; g < nblk; g++) {
+        const double* o = part + ((size_t)g * bins + bn) * 4;
+        kadd(sP, cP, o[0]);
+        kadd(sP, cP, o[1]);
+        kadd(sK, cK, o[2]);
+        kadd(sK, cK, o[3]);
+        c += cnt[(size_t)g * bins + bn];
+    }
+    out[bn * 2 + 0] = sP + cP;
+    out[bn * 2 + 1] = sK + cK;
+    obs[bn] = c;
+}
+
+// Host: device LUT of the reference bin index per integer n^2 = nx^2+ny^2+nz^2.
+const int* pk_bin_lut(int N, double boxlen, int bins);
+// Host: number of work-groups used by k_pk_partial
+inline int pk_blocks(long nmodes) {
+    long b = (nmodes + 255) / 256;
+    return (int)(b < 1024 ? b : 1024);
+}
+
+// Binned P(k) of two half-complex grids with row stride hs; host outputs
+// sums (P, k) and counts.  `xf` transforms each value before the product.
+template <typename T, typename XF>
+inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen, int bins, XF xf,
+           double* sums, long long* obs, hipStream_t s, double* dev_sums, long long* dev_obs) {
+    const int* lut = pk_bin_lut(N, boxlen, bins);
+    if (!lut) return MITOS_ERR_ALLOC;
+    const long nmodes = (long)N * N * (N / 2 + 1);
+    const int nblk = pk_blocks(nmodes);
+    const size_t pbytes = sizeof(double) * 4 * bins * nblk;
+    const size_t cbytes = sizeof(long long) * bins * nblk;
+    char* ws = (char*)workspace(2, pbytes + cbytes + sizeof(double) * 2 * bins +
+                                       sizeof(long long) * bins);
+    if (!ws) return MITOS_ERR_ALLOC;
+    double* part = (double*)ws;
+    long long* cnt = (long long*)(ws + pbytes);
+    double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes + cbytes);
+    long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + cbytes + sizeof(double) * 2 * bins);
+    const size_t shm = sizeof(double) * 4 * 4 * bins + sizeof(long long) * 4 * bins;
+    const double dk = 2 * M_PI / boxlen;
+    hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, dk,
+                       lut, bins, part, cnt, xf);
+    MITOS_TRY(launch_status());
+    hipLaunchKernelGGL(k_pk_final, dim3((bins + 63) / 64), dim3(64), 0, s, part, cnt, nblk,
+                       bins, fs, fo);
+    MITOS_TRY(launch_status());
+    if (sums) {
+        MITOS_HIP_CHECK(hipMemcpyAsync(sums, fs, sizeof(double) * 2 * bins,
+                                       hipMemcpyDeviceToHost, s));
+        MITOS_HIP_CHECK(hipMemcpyAsync(obs, fo, sizeof(long long) * bins, hipMemcpyDeviceToHost,
+                                       s));
+        MITOS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return MITOS_OK;
+}
+
+// src/calc_powerspec.c:84-89 on the host: averages, empty bins give 0/0 = NaN
+void pk_finish(const double* sums, const long long* obs, int bins, double boxlen, double* k_out,
+               double* p_out);
+
+}  // namespace mitos

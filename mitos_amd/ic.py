@@ -1,0 +1,289 @@
+"""Python mirror of the reference's IC call surface over libmitos_hip.so.
+
+Names follow the reference (``alloc_local_grid``, ``generate_complex_grf``,
+``fft_r2c_dg``, ``solve2LPT``, ``calc_cross_powerspec`` ...) with the same
+argument meaning and return codes; grids live in GPU memory.  Device arrays
+passed to the serial-array functions are torch CUDA tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import MITOS_FP32, MITOS_FP64, MitosError, check, kernel_ptr
+
+
+def lib():
+    return _lib.load()
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+class Grid:
+    """A ``struct distributed_grid`` on one rank (NX = N, X0 = 0) in device memory."""
+
+    def __init__(self, N: int, boxlen: float):
+        self.s = _lib.DistributedGrid()
+        check(lib().alloc_local_grid(ctypes.byref(self.s), N, boxlen, 0x44000000),
+              "alloc_local_grid")
+
+    @property
+    def N(self) -> int:
+        return self.s.N
+
+    @property
+    def boxlen(self) -> float:
+        return self.s.boxlen
+
+    @property
+    def momentum_space(self) -> int:
+        return ord(self.s.momentum_space)
+
+    @momentum_space.setter
+    def momentum_space(self, v: int) -> None:
+        self.s.momentum_space = bytes([v])
+
+    def ref(self):
+        return ctypes.byref(self.s)
+
+    def fbox(self) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N // 2 + 1), np.complex128)
+        check(lib().mitos_grid_to_host(self.ref(), out.ctypes.data, None), "mitos_grid_to_host")
+        return out
+
+    def box(self) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N + 2), np.float64)
+        check(lib().mitos_grid_to_host(self.ref(), None, out.ctypes.data), "mitos_grid_to_host")
+        return out
+
+    def set_fbox(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, np.complex128)
+        check(lib().mitos_grid_from_host(self.ref(), a.ctypes.data, None), "mitos_grid_from_host")
+
+    def set_box(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, np.float64)
+        check(lib().mitos_grid_from_host(self.ref(), None, a.ctypes.data), "mitos_grid_from_host")
+
+    def free(self) -> None:
+        if self.s.fbox or self.s.box:
+            lib().free_local_grid(self.ref())
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rand_uint64_init(seed: int):
+    return lib().rand_uint64_init(seed)
+
+
+def generate_complex_grf(g: Grid, state) -> int:
+    return lib().generate_complex_grf(g.ref(), ctypes.byref(state))
+
+
+def enforce_hermiticity(g: Grid) -> int:
+    return lib().enforce_hermiticity(g.ref())
+
+
+def fft_r2c_dg(g: Grid) -> int:
+    return lib().fft_r2c_dg(g.ref())
+
+
+def fft_c2r_dg(g: Grid) -> int:
+    return lib().fft_c2r_dg(g.ref())
+
+
+def kernel_params(name: str, **kw):
+    """ctypes parameter block for a kernel callback (kept alive by the caller)."""
+    if name == "kernel_constant":
+        return ctypes.c_double(kw["constant"])
+    if name == "kernel_power_no_transfer":
+        c = _lib.Cosmology()
+        c.A_s, c.n_s, c.k_pivot = kw["A_s"], kw["n_s"], kw["k_pivot"]
+        return c
+    if name == "kernel_undo_Hermite_window":
+        return _lib.HermiteKernParams(kw["order"], kw["N"], kw["boxlen"])
+    if name in ("kernel_sqrt_power_law", "kernel_bbks_transfer"):
+        return _lib.SpectrumParams(kw.get("A", 0.0), kw.get("n_s", 0.0), kw.get("Gamma", 0.0))
+    return None
+
+
+def fft_apply_kernel_dg(w: Grid, r: Grid, name: str, params=None) -> int:
+    p = ctypes.cast(ctypes.byref(params), ctypes.c_void_p) if params is not None else None
+    return lib().fft_apply_kernel_dg(w.ref(), r.ref(), kernel_ptr(name), p)
+
+
+def solvePoisson_dg(g: Grid) -> int:
+    return lib().solvePoisson_dg(g.ref())
+
+
+def solve2LPT(pot: Grid, dens: Grid, work: Grid, f1: float, f2: float) -> int:
+    return lib().solve2LPT(pot.ref(), dens.ref(), work.ref(), f1, f2)
+
+
+def calc_cross_powerspec(N: int, boxlen: float, box1, box2, bins: int, fp32: bool = False):
+    """box1/box2: torch CUDA tensors in the (N, N, N/2+1) complex layout."""
+    _sync()
+    k = np.empty(bins)
+    P = np.empty(bins)
+    obs = np.empty(bins, np.int32)
+    fn = lib().calc_cross_powerspec_float if fp32 else lib().calc_cross_powerspec
+    fn(N, boxlen, _ptr(box1), _ptr(box2), bins, k.ctypes.data_as(_lib.c_double_p),
+       P.ctypes.data_as(_lib.c_double_p), obs.ctypes.data_as(_lib.c_int_p))
+    check(lib().mitos_last_error(), "calc_cross_powerspec")
+    return k, P, obs
+
+
+def sigma8_amplitude(n_s: float, gamma: float, sigma8: float, R: float = 8.0) -> float:
+    return lib().mitos_sigma8_amplitude(n_s, gamma, sigma8, R)
+
+
+def pk_bin_table(N: int, boxlen: float, bins: int) -> np.ndarray:
+    t = np.empty(3 * (N // 2) ** 2 + 1, np.int32)
+    check(lib().mitos_pk_bin_table(N, boxlen, bins, t.ctypes.data), "mitos_pk_bin_table")
+    return t
+
+
+@dataclass
+class ICConfig:
+    """struct mitos_ic_config (the .ini keys of the path)."""
+    N: int = 64
+    boxlen: float = 256.0
+    seed: int = 42
+    n_s: float = 0.965
+    sigma8: float = 0.81
+    gamma: float = 0.21
+    f1: float = 1.0
+    f2: float = -3.0 / 7.0
+    run2lpt: int = 1
+    bins: int = 32
+    precision: int = MITOS_FP64
+
+    def c(self) -> _lib.IcConfig:
+        return _lib.IcConfig(self.N, self.boxlen, self.seed, self.n_s, self.sigma8, self.gamma,
+                             self.f1, self.f2, self.run2lpt, self.bins, self.precision)
+
+    @staticmethod
+    def from_ini(path: str) -> "ICConfig":
+        c = _lib.IcConfig()
+        check(lib().mitos_read_ini(path.encode(), ctypes.byref(c)), "mitos_read_ini")
+        return ICConfig(c.N, c.boxlen, c.seed, c.n_s, c.sigma8, c.gamma, c.f1, c.f2, c.run2lpt,
+                        c.bins, c.precision)
+
+
+class IC:
+    """A fused-pipeline context (mitos_ic_create / run / destroy)."""
+
+    def __init__(self, cfg: ICConfig):
+        self.cfg = cfg
+        self.h = ctypes.c_void_p()
+        check(lib().mitos_ic_create(ctypes.byref(cfg.c()), ctypes.byref(self.h)),
+              "mitos_ic_create")
+
+    def run(self, cfg: ICConfig | None = None) -> dict:
+        cfg = cfg or self.cfg
+        bins = cfg.bins
+        k = np.empty(bins)
+        P = np.empty(bins)
+        obs = np.empty(bins, np.int64)
+        res = _lib.IcResult(k.ctypes.data_as(_lib.c_double_p), P.ctypes.data_as(_lib.c_double_p),
+                            obs.ctypes.data_as(_lib.c_ll_p), 0.0)
+        check(lib().mitos_ic_run(self.h, ctypes.byref(cfg.c()), ctypes.byref(res)),
+              "mitos_ic_run")
+        return {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude,
+                "stage_ms": list(res.stage_ms)}
+
+    def enqueue(self, cfg: ICConfig | None = None, stream: int | None = None) -> None:
+        cfg = cfg or self.cfg
+        check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
+
+    def positions(self):
+        import torch
+        n = self.cfg.N ** 3
+        X = torch.empty(n, dtype=torch.float32, device="cuda")
+        Y = torch.empty_like(X)
+        Z = torch.empty_like(X)
+        _sync()
+        check(lib().mitos_ic_positions(self.h, _ptr(X), _ptr(Y), _ptr(Z)), "mitos_ic_positions")
+        return X, Y, Z
+
+    def _real(self):
+        import torch
+        return torch.float32 if self.cfg.precision == MITOS_FP32 else torch.float64
+
+    def displacements(self):
+        import torch
+        N = self.cfg.N
+        out = torch.empty((3, N, N, N), dtype=self._real(), device="cuda")
+        _sync()
+        check(lib().mitos_ic_displacements(self.h, _ptr(out)), "mitos_ic_displacements")
+        return out
+
+    def density(self):
+        import torch
+        N = self.cfg.N
+        out = torch.empty((N, N, N), dtype=self._real(), device="cuda")
+        _sync()
+        check(lib().mitos_ic_density(self.h, _ptr(out)), "mitos_ic_density")
+        return out
+
+    def ids(self):
+        import torch
+        out = torch.empty(self.cfg.N ** 3, dtype=torch.int64, device="cuda")
+        _sync()
+        check(lib().mitos_ic_ids(self.h, _ptr(out)), "mitos_ic_ids")
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            lib().mitos_ic_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_reference_order(cfg: ICConfig, positions: bool = False) -> dict:
+    """The reference's call order through the drop-in functions (FP64 plain path)."""
+    bins = cfg.bins
+    k = np.empty(bins)
+    P = np.empty(bins)
+    obs = np.empty(bins, np.int64)
+    res = _lib.IcResult(k.ctypes.data_as(_lib.c_double_p), P.ctypes.data_as(_lib.c_double_p),
+                        obs.ctypes.data_as(_lib.c_ll_p), 0.0)
+    X = Y = Z = None
+    ptrs = (None, None, None)
+    if positions:
+        import torch
+        n = cfg.N ** 3
+        X = torch.empty(n, dtype=torch.float32, device="cuda")
+        Y = torch.empty_like(X)
+        Z = torch.empty_like(X)
+        _sync()
+        ptrs = (_ptr(X), _ptr(Y), _ptr(Z))
+    c = cfg.c()
+    c.precision = MITOS_FP64
+    check(lib().mitos_ic_run_reference_order(ctypes.byref(c), ctypes.byref(res), *ptrs),
+          "mitos_ic_run_reference_order")
+    out = {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude,
+           "stage_ms": list(res.stage_ms)}
+    if positions:
+        out.update(X=X, Y=Y, Z=Z)
+    return out

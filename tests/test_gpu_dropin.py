@@ -1,0 +1,243 @@
+"""GPU parity of the drop-in (reference-named) entry points against the oracle.
+
+Each test calls libmitos_hip.so through its C ABI and compares with the FP64
+CPU oracle on the same seeded inputs.  Integer/stream outputs are compared
+bit for bit; floating-point fields with the norm-relative tolerance of the
+FP64 contract (1e-10) unless stated.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [8, 16, 32, 64]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def test_fft_r2c_c2r_dg(gpu, oracle):
+    rng = np.random.default_rng(1)
+    for N in SIZES + [128]:
+        print("case fft_dg N", N, flush=True)
+        L = 100.0 + N
+        g = gpu.Grid(N, L)
+        box = np.zeros((N, N, N + 2))
+        box[:, :, :N] = rng.standard_normal((N, N, N))
+        g.set_box(box)
+        assert gpu.fft_r2c_dg(g) == 0
+        assert g.momentum_space == 1
+        ref = oracle.fft_r2c(box[:, :, :N]) * (L ** 3 / N ** 3)
+        assert rel(g.fbox(), ref) < 1e-13
+        # non-Hermitian input: c2r keeps Im of z'=0,N/2 out, as the oracle's c2r
+        f = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+        g.set_fbox(f)
+        assert gpu.fft_c2r_dg(g) == 0
+        assert g.momentum_space == 0
+        out = g.box()
+        refr = oracle.fft_c2r(f) / L ** 3
+        assert rel(out[:, :, :N], refr) < 1e-13
+        assert np.all(out[:, :, N:] == 0)
+        g.free()
+
+
+def test_serial_fft_float(gpu, oracle):
+    import torch
+    rng = np.random.default_rng(2)
+    lib = gpu.lib()
+    for N in [16, 64, 256]:
+        print("case fft_float N", N, flush=True)
+        a = rng.standard_normal((N, N, N)).astype(np.float32)
+        da = torch.from_numpy(a).cuda()
+        out = torch.empty((N, N, N // 2 + 1), dtype=torch.complex64, device="cuda")
+        torch.cuda.synchronize()
+        assert lib.mitos_fft_r2c_float(N, da.data_ptr(), out.data_ptr()) == 0
+        ref = np.fft.rfftn(a.astype(np.float64))
+        assert rel(out.cpu().numpy(), ref) < 2e-6
+        back = torch.empty((N, N, N), dtype=torch.float32, device="cuda")
+        assert lib.mitos_fft_c2r_float(N, out.data_ptr(), back.data_ptr()) == 0
+        assert rel(back.cpu().numpy() / N ** 3, a) < 2e-6
+
+
+def test_apply_kernels(gpu, oracle):
+    rng = np.random.default_rng(3)
+    N, L = 32, 150.0
+    f = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    cases = [
+        ("kernel_inv_poisson", {}, oracle.K_INV_POISSON, ()),
+        ("kernel_dx", {}, oracle.K_DX, ()),
+        ("kernel_dy", {}, oracle.K_DY, ()),
+        ("kernel_dz", {}, oracle.K_DZ, ()),
+        ("kernel_constant", {"constant": 2.5}, oracle.K_CONSTANT, (2.5,)),
+        ("kernel_power_no_transfer", {"A_s": 2.1e-9, "n_s": 0.96, "k_pivot": 0.05},
+         oracle.K_POWER_NO_TRANSFER, (2.1e-9, 0.96, 0.05)),
+        ("kernel_undo_Hermite_window", {"order": 3, "N": N, "boxlen": L}, oracle.K_UNDO_WINDOW,
+         (3, N, L)),
+        ("kernel_sqrt_power_law", {"A": 1.5e6, "n_s": 0.965}, oracle.K_SQRT_POWER_LAW,
+         (1.5e6, 0.965)),
+        ("kernel_bbks_transfer", {"Gamma": 0.21}, oracle.K_BBKS, (0.21,)),
+    ]
+    r = gpu.Grid(N, L)
+    w = gpu.Grid(N, L)
+    for name, kw, kid, prm in cases:
+        print("case kernel", name, flush=True)
+        r.set_fbox(f)
+        r.momentum_space = 1
+        p = gpu.kernel_params(name, **kw)
+        assert gpu.fft_apply_kernel_dg(w, r, name, p) == 0
+        ref = oracle.apply_kernel(f, L, kid, prm)
+        assert rel(w.fbox(), ref) < 1e-14, name
+        assert w.momentum_space == 1
+    # error codes of src/fft.c:269-277
+    r.momentum_space = 0
+    assert gpu.fft_apply_kernel_dg(w, r, "kernel_dx") == 2
+    other = gpu.Grid(16, L)
+    other.momentum_space = 1
+    assert gpu.fft_apply_kernel_dg(w, other, "kernel_dx") == 1
+
+
+def test_unknown_callback_rejected(gpu):
+    import ctypes
+    from mitos_amd import _lib
+    N, L = 16, 50.0
+    r = gpu.Grid(N, L)
+    w = gpu.Grid(N, L)
+    r.momentum_space = 1
+
+    @_lib.KERNEL_FN
+    def mine(k):  # a host callback the device cannot evaluate
+        pass
+    code = gpu.lib().fft_apply_kernel_dg(w.ref(), r.ref(), ctypes.cast(mine, ctypes.c_void_p),
+                                         None)
+    assert code == 4
+
+
+def test_grf_stream_and_hermiticity(gpu, oracle):
+    for N, seed in [(8, 42), (16, 100), (32, 7), (64, 42)]:
+        print("case grf N", N, "seed", seed, flush=True)
+        L = 256.0
+        g = gpu.Grid(N, L)
+        st = gpu.rand_uint64_init(seed)
+        assert gpu.generate_complex_grf(g, st) == 0
+        assert g.momentum_space == 1
+        got = g.fbox()
+        ref = oracle.grf(N, L, seed, hermitian=False)
+        # state advanced by exactly the draws the reference consumes
+        nd = 4 * (N * N * (N // 2 + 1) - 1)
+        assert list(st.s) == [int(v) for v in _state_after(oracle, seed, nd)]
+        same = np.mean(got.view(np.uint64) == ref.view(np.uint64))
+        print("  bit-identical fraction", same, flush=True)
+        assert rel(got, ref) < 1e-15
+        assert same > 0.99
+        assert gpu.enforce_hermiticity(g) == 0
+        refh = oracle.grf(N, L, seed, hermitian=True)
+        goth = g.fbox()
+        assert rel(goth, refh) < 1e-15
+        for z in (0, N // 2):
+            P = goth[:, :, z]
+            M = np.conj(P[(-np.arange(N)) % N][:, (-np.arange(N)) % N])
+            assert np.array_equal(P, M)
+        g.free()
+
+
+def _state_after(oracle, seed, ndraws):
+    return oracle.state_after(seed, ndraws)
+
+
+def test_solve2lpt(gpu, oracle):
+    N, L = 32, 200.0
+    print("case solve2LPT N", N, flush=True)
+    f = oracle.grf(N, L, 11)
+    f = oracle.apply_kernel(f, L, oracle.K_SQRT_POWER_LAW, (2e5, 0.965))
+    # oracle
+    od, op, ow = oracle.Grid(N, L), oracle.Grid(N, L), oracle.Grid(N, L)
+    od.fbox[:] = f
+    od.box[:, :, :N] = oracle.fft_c2r(f) / L ** 3
+    od.s.momentum_space = b"\x01"
+    assert oracle.solve_2lpt(op, od, ow, 1.0, -3.0 / 7.0) == 0
+    # device
+    d, p, w = gpu.Grid(N, L), gpu.Grid(N, L), gpu.Grid(N, L)
+    d.set_fbox(f)
+    d.set_box(od.box)
+    d.momentum_space = 1
+    assert gpu.solve2LPT(p, d, w, 1.0, -3.0 / 7.0) == 0
+    assert p.momentum_space == 1
+    assert rel(p.fbox(), op.fbox) < 1e-11
+    assert rel(w.box()[:, :, :N], ow.box[:, :, :N]) < 1e-11
+    # error codes: density in configuration space, aliased grids
+    d.momentum_space = 0
+    assert gpu.solve2LPT(p, d, w, 1.0, -3.0 / 7.0) == 2
+    d.momentum_space = 1
+    assert gpu.solve2LPT(p, d, p, 1.0, -3.0 / 7.0) == 5
+
+
+def test_particles_tsc_cic(gpu, oracle):
+    import torch
+    N, L = 32, 256.0
+    print("case particles N", N, flush=True)
+    rng = np.random.default_rng(5)
+    lib = gpu.lib()
+    psi = np.zeros((3, N, N, N + 2))
+    psi[:, :, :, :N] = 3.0 * rng.standard_normal((3, N, N, N))
+    n = N ** 3
+    X, Y, Z, ids = oracle.lattice(N, L)
+    dX, dY, dZ = (torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(3))
+    dI = torch.empty(n, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    assert lib.mitos_lattice(N, L, 1, dX.data_ptr(), dY.data_ptr(), dZ.data_ptr(),
+                             dI.data_ptr()) == 0
+    assert np.array_equal(dX.cpu().numpy(), X) and np.array_equal(dZ.cpu().numpy(), Z)
+    assert np.array_equal(dI.cpu().numpy(), ids)
+    grids = []
+    for i in range(3):
+        g = gpu.Grid(N, L)
+        g.set_box(psi[i])
+        g.momentum_space = 0
+        grids.append(g)
+    assert lib.mitos_displace_tsc_dg(grids[0].ref(), grids[1].ref(), grids[2].ref(), n,
+                                     dX.data_ptr(), dY.data_ptr(), dZ.data_ptr()) == 0
+    oracle.displace_tsc(psi, L, X, Y, Z)
+    gx, gy, gz = dX.cpu().numpy(), dY.cpu().numpy(), dZ.cpu().numpy()
+    print("  position identical fraction", np.mean(gx == X), np.mean(gz == Z), flush=True)
+    assert np.abs(gx - X).max() <= 2e-4 and np.abs(gz - Z).max() <= 2e-4
+    assert np.mean((gx == X) & (gy == Y) & (gz == Z)) > 0.999
+    # CIC overdensity
+    box, tot = oracle.cic(N, L, X, Y, Z, 1.0)
+    oracle.overdensity(box, L, tot)
+    dbox = torch.empty((N, N, N), dtype=torch.float64, device="cuda")
+    Xd, Yd, Zd = (torch.from_numpy(a).cuda() for a in (X, Y, Z))
+    torch.cuda.synchronize()
+    assert lib.mitos_cic_overdensity(N, L, n, Xd.data_ptr(), Yd.data_ptr(), Zd.data_ptr(),
+                                     dbox.data_ptr()) == 0
+    got = dbox.cpu().numpy()
+    assert np.abs(got - box).max() < 1e-10
+    assert abs(got.sum()) < 1e-6  # mass conservation: mean overdensity is zero
+
+
+def test_calc_cross_powerspec(gpu, oracle):
+    import torch
+    rng = np.random.default_rng(7)
+    for N, L, bins in [(16, 100.0, 8), (32, 256.0, 16), (64, 500.0, 32)]:
+        print("case pk N", N, flush=True)
+        a = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+        b = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+        da = torch.from_numpy(a).cuda()
+        db = torch.from_numpy(b).cuda()
+        k, P, obs = gpu.calc_cross_powerspec(N, L, da, db, bins)
+        rk, rP, robs = oracle.powerspec(a, b, L, bins)
+        assert np.array_equal(obs, robs)
+        ok = robs > 0
+        assert np.all(np.isnan(k[~ok])) and np.all(np.isnan(P[~ok]))
+        assert np.allclose(k[ok], rk[ok], rtol=1e-13, atol=0)
+        assert np.allclose(P[ok], rP[ok], rtol=1e-11, atol=1e-300)
+        # determinism: repeated calls give identical bits
+        k2, P2, _ = gpu.calc_cross_powerspec(N, L, da, db, bins)
+        assert np.array_equal(P[ok], P2[ok]) and np.array_equal(k[ok], k2[ok])
+        # float variant
+        fa, fb = da.to(torch.complex64), db.to(torch.complex64)
+        kf, Pf, obsf = gpu.calc_cross_powerspec(N, L, fa, fb, bins, fp32=True)
+        rkf, rPf, _ = oracle.powerspec(a.astype(np.complex64).astype(np.complex128),
+                                       b.astype(np.complex64).astype(np.complex128), L, bins)
+        assert np.array_equal(obsf, robs)
+        assert np.allclose(Pf[ok], rPf[ok], rtol=1e-11, atol=1e-300)

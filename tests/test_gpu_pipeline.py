@@ -1,0 +1,96 @@
+"""GPU parity of the whole IC pipeline (fused FP64/FP32 and the reference-order
+FP64 plain path) against the oracle's reference-order FP64 run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(oracle, N, L, seed, bins, run2lpt=True):
+    return oracle.ic_run(N, L, seed, bins=bins, run2lpt=run2lpt, want_psi=True,
+                         want_density=True)
+
+
+def _pk_rel(P, ref, obs):
+    ok = obs > 1
+    return np.abs(P[ok] - ref[ok]) / np.abs(ref[ok])
+
+
+def _pos_err(X, Xr, L):
+    d = np.abs(X.astype(np.float64) - Xr.astype(np.float64))
+    d = np.minimum(d, L - d)  # periodic
+    return d.max()
+
+
+@pytest.mark.parametrize("N,L,seed,bins,run2lpt", [
+    (32, 128.0, 42, 16, True), (64, 256.0, 42, 32, True), (32, 256.0, 3, 16, False)])
+def test_fused_fp64_vs_oracle(gpu, oracle, N, L, seed, bins, run2lpt):
+    print("case fused fp64", N, L, seed, bins, run2lpt, flush=True)
+    ref = _oracle_run(oracle, N, L, seed, bins, run2lpt)
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, run2lpt=int(run2lpt),
+                       precision=gpu.MITOS_FP64)
+    ic = gpu.IC(cfg)
+    out = ic.run()
+    assert np.array_equal(out["obs"], ref["obs"])
+    err = _pk_rel(out["P"], ref["P"], ref["obs"])
+    print("  P(k) max rel err", err.max(), flush=True)
+    assert err.max() < 1e-10
+    psi = ic.displacements().cpu().numpy()
+    rpsi = ref["psi"][:, :, :, :N]
+    e = np.linalg.norm(psi - rpsi) / np.linalg.norm(rpsi)
+    print("  psi rel err", e, flush=True)
+    assert e < 1e-12
+    X, Y, Z = (t.cpu().numpy() for t in ic.positions())
+    pe = max(_pos_err(X, ref["X"], L), _pos_err(Y, ref["Y"], L), _pos_err(Z, ref["Z"], L))
+    print("  max position diff", pe, flush=True)
+    assert pe < 1e-3 * L / N
+    dens = ic.density().cpu().numpy()
+    de = np.linalg.norm(dens - ref["density"]) / np.linalg.norm(ref["density"])
+    print("  density rel err", de, flush=True)
+    assert de < 1e-6
+    ids = ic.ids().cpu().numpy()
+    assert np.array_equal(ids, ref["id"])
+    ic.close()
+
+
+@pytest.mark.parametrize("N,L,seed,bins", [(32, 128.0, 42, 16), (64, 256.0, 42, 32)])
+def test_fused_fp32_vs_oracle(gpu, oracle, N, L, seed, bins):
+    print("case fused fp32", N, L, seed, bins, flush=True)
+    ref = _oracle_run(oracle, N, L, seed, bins)
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=gpu.MITOS_FP32)
+    ic = gpu.IC(cfg)
+    out = ic.run()
+    assert np.array_equal(out["obs"], ref["obs"])
+    err = _pk_rel(out["P"], ref["P"], ref["obs"])
+    print("  P(k) max rel err (fp32)", err.max(), flush=True)
+    assert err.max() < 1e-4
+    psi = ic.displacements().cpu().numpy().astype(np.float64)
+    rpsi = ref["psi"][:, :, :, :N]
+    e = np.linalg.norm(psi - rpsi) / np.linalg.norm(rpsi)
+    print("  psi rel err (fp32)", e, flush=True)
+    assert e < 1e-5
+    ic.close()
+
+
+def test_reference_order_fp64_vs_oracle(gpu, oracle):
+    N, L, seed, bins = 32, 256.0, 42, 16
+    print("case reference order", N, flush=True)
+    ref = _oracle_run(oracle, N, L, seed, bins)
+    out = gpu.run_reference_order(gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins),
+                                  positions=True)
+    assert np.array_equal(out["obs"], ref["obs"])
+    err = _pk_rel(out["P"], ref["P"], ref["obs"])
+    print("  P(k) max rel err", err.max(), flush=True)
+    assert err.max() < 1e-10
+    X = out["X"].cpu().numpy()
+    assert _pos_err(X, ref["X"], L) < 1e-3 * L / N
+
+
+def test_fused_repeatable(gpu):
+    cfg = gpu.ICConfig(N=32, boxlen=100.0, seed=5, bins=16, precision=gpu.MITOS_FP32)
+    ic = gpu.IC(cfg)
+    a = ic.run()
+    b = ic.run()
+    ok = a["obs"] > 0
+    assert np.array_equal(a["P"][ok], b["P"][ok])
+    ic.close()
