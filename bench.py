@@ -1,0 +1,204 @@
+"""Benchmark: IC particles/s at 1024^3 (GRF + 2LPT + TSC displacement + CIC + P(k)).
+
+One step = one realization of the fused FP32 pipeline (white noise from seed
+42, BBKS spectrum normalised to sigma8 = 0.81, 2LPT, 1024^3 lattice particles,
+CIC deposit, P(k) in 128 bins), box 2000 Mpc/h, all inputs resident in HBM.
+The same realization is also run through the plain FP64 path (the reference's
+call order, one drop-in call per step) and the fused FP64 path on the same
+GPU, and the FP32 P(k)/displacements are compared with the FP64 ones.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 1024]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "IC particles/s at 1024^3 (GRF+2LPT+CIC+P(k)), FP32 vs FP64 plain, same GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def b_alg(N: int, s: int) -> float:
+    """Algorithmic bytes of one realization: s (28 N^3 + 60 N^2 (N+2))."""
+    return s * (28.0 * N ** 3 + 60.0 * N ** 2 * (N + 2))
+
+
+def cpu_baseline(min_seconds: float = 10.0) -> dict:
+    """The oracle's reference-order FP64 pipeline on one host core."""
+    from oracle import oracle as O
+    O.build()
+    n, L = 128, 250.0  # 1.95 Mpc/h cells, as 1024^3 in 2000 Mpc/h
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.ic_run(n, L, 42, bins=32, threads=1)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds:
+            break
+    return {"value": reps * n ** 3 / el, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 reference-order pipeline (GRF, 19 FFTs, 2LPT, TSC, CIC, "
+                      f"P(k) 32 bins) at {n}^3, L={L} Mpc/h, seed 42, {reps} runs in {el:.1f} s, "
+                      "1 thread"}
+
+
+def pmc_pass(ic, torch, N, L, bins):
+    """Dispatches profiled by the counter runs: realization 2 is the measured one."""
+    cfg = ic.ICConfig(N=N, boxlen=L, seed=42, bins=bins, precision=ic.MITOS_FP32)
+    ctx = ic.IC(cfg)
+    stream = torch.cuda.Stream()
+    for i in range(2):
+        ctx.enqueue(cfg, stream.cuda_stream)
+        torch.cuda.synchronize()
+        log(f"[pmc] realization {i + 1} done")
+    ctx.close()
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)  # copy calibration kernel: reads 1 GiB, writes 1 GiB
+    torch.cuda.synchronize()
+    log("[pmc] copy calibration done (1 GiB read, 1 GiB written)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--boxlen", type=float, default=2000.0)
+    ap.add_argument("--bins", type=int, default=128)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparisons")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="one warm-up and one measured FP32 realization, then a plain copy "
+                         "of 1 GiB (counter calibration); for rocprofv3 --pmc runs")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU (requested --gpus {a.gpus}); this build has no "
+              "multi-GPU path: the slab/pencil sharding is a design only.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import mitos_amd.ic as ic
+
+    N, L, bins = a.N, a.boxlen, a.bins
+    if a.pmc_pass:
+        pmc_pass(ic, torch, N, L, bins)
+        return
+    out = {"metric": METRIC, "unit": "particles/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    out["config"] = {"workload": f"Mitos IC realization {N}^3, box {L:g} Mpc/h, seed 42, BBKS "
+                                 f"Gamma=0.21 n_s=0.965 sigma8=0.81, 2LPT, TSC displacement, "
+                                 f"CIC, P(k) {bins} bins; FP32 fields, FP64 P(k) sums",
+                     "N": N, "boxlen": L, "seed": 42, "bins": bins, "precision": "fp32",
+                     "pipeline": "fused"}
+
+    # ---- FP32 fused: the timed path -------------------------------------------------------
+    cfg = ic.ICConfig(N=N, boxlen=L, seed=42, bins=bins, precision=ic.MITOS_FP32)
+    log(f"[bench] fp32 fused {N}^3: create")
+    ctx = ic.IC(cfg)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    for i in range(a.warmup):
+        ctx.enqueue(cfg, sp)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done; timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.enqueue(cfg, sp)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = 1e3 * el / a.steps
+    out["value"] = N ** 3 * a.steps / el
+    out["ms_per_step"] = ms
+    # one more run with the per-kernel profile and the P(k) result
+    ctx.profile(True)
+    r32 = ctx.run(cfg)
+    prof = ctx.profile_entries()
+    ctx.profile(False)
+    tot = {}
+    for name, kms, by in prof:
+        t = tot.setdefault(name, [0.0, 0.0, 0])
+        t[0] += kms
+        t[1] += by
+        t[2] += 1
+    dom = max(tot.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dby, dn) = dom
+    achieved = dby / (dms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                       "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                       "kernel": dname, "launches": dn, "kernel_ms": dms / dn,
+                       "algorithmic_bytes_per_launch": dby / dn}
+    out["kernels"] = {k: {"ms": v[0], "GB_per_s": v[1] / (v[0] * 1e-3) / 1e9 if v[0] else None,
+                          "launches": v[2]} for k, v in tot.items()}
+    pipe_frac = b_alg(N, 4) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS
+    out["pipeline_roofline"] = {"algorithmic_bytes": b_alg(N, 4), "achieved_GBs":
+                                b_alg(N, 4) / (ms * 1e-3) / 1e9, "frac": pipe_frac,
+                                "formula": "s*(28 N^3 + 60 N^2 (N+2)), s=4"}
+    log(f"[bench] fp32: {ms:.1f} ms/step, {out['value']:.3e} particles/s; dominant {dname}")
+    psi32 = None
+    if not a.no_fp64:
+        psi32 = ctx.displacements()
+    ctx.close()
+    del ctx
+
+    if not a.no_fp64:
+        # ---- FP64 fused and plain on the same realization ----------------------------------
+        cfg64 = ic.ICConfig(N=N, boxlen=L, seed=42, bins=bins, precision=ic.MITOS_FP64)
+        log("[bench] fp64 fused")
+        c64 = ic.IC(cfg64)
+        c64.run(cfg64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r64 = c64.run(cfg64)
+        el64 = time.perf_counter() - t0
+        psi64 = c64.displacements()
+        c64.close()
+        del c64
+        d = (psi32.double() - psi64)
+        rms = float(torch.sqrt((d * d).mean()) / torch.sqrt((psi64 * psi64).mean()))
+        mx = float(d.abs().max() / psi64.abs().max())
+        del d, psi32, psi64
+        torch.cuda.empty_cache()
+        ok = r64["obs"] > 1
+        pk_err = np.abs(r32["P"][ok] - r64["P"][ok]) / np.abs(r64["P"][ok])
+        log("[bench] fp64 plain (reference call order)")
+        t0 = time.perf_counter()
+        rp = ic.run_reference_order(cfg64)
+        elp = time.perf_counter() - t0
+        okp = rp["obs"] > 1
+        pk_plain = np.abs(rp["P"][okp] - r64["P"][okp]) / np.abs(r64["P"][okp])
+        out["fp64_plain"] = {"particles_per_s": N ** 3 / elp, "ms": 1e3 * elp,
+                             "pipeline": "reference call order, drop-in calls, 19 FFTs"}
+        out["fp64_fused"] = {"particles_per_s": N ** 3 / el64, "ms": 1e3 * el64}
+        out["speedup_fp32_vs_fp64_plain"] = out["value"] / out["fp64_plain"]["particles_per_s"]
+        out["precision"] = {
+            "pk_max_rel_err_fp32_vs_fp64": float(pk_err.max()),
+            "pk_rms_rel_err_fp32_vs_fp64": float(np.sqrt((pk_err ** 2).mean())),
+            "displacement_rms_rel_err_fp32_vs_fp64": rms,
+            "displacement_max_err_over_max_fp32_vs_fp64": mx,
+            "pk_max_rel_diff_fp64_fused_vs_plain": float(pk_plain.max()),
+            "bins_compared": int(ok.sum())}
+        log(f"[bench] fp64 fused {1e3 * el64:.1f} ms, plain {1e3 * elp:.1f} ms; "
+            f"P(k) fp32 err {pk_err.max():.2e}")
+
+    if not a.no_cpu:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -307,6 +307,13 @@ int mitos_ic_displacements(mitos_ic_context *ctx, void *psi_device);
 int mitos_ic_ids(mitos_ic_context *ctx, long long *ids_device);
 int mitos_ic_destroy(mitos_ic_context *ctx);
 
+/* per-kernel profile of the following runs (HIP events on the run's stream):
+ * mitos_ic_profile_entry returns the number of entries and fills entry i
+ * (kernel name, milliseconds, algorithmic bytes) when 0 <= i < count */
+int mitos_ic_profile(mitos_ic_context *ctx, int enable);
+int mitos_ic_profile_entry(mitos_ic_context *ctx, int i, const char **name, float *ms,
+                           double *bytes);
+
 /* the reference's call order through the Part 1 functions (FP64 plain path) */
 int mitos_ic_run_reference_order(const struct mitos_ic_config *cfg,
                                  struct mitos_ic_result *res, float *X, float *Y, float *Z);

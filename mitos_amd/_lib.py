@@ -156,6 +156,10 @@ SIGNATURES = [
     ("mitos_ic_displacements", ctypes.c_int, [vp, vp]),
     ("mitos_ic_ids", ctypes.c_int, [vp, vp]),
     ("mitos_ic_destroy", ctypes.c_int, [vp]),
+    ("mitos_ic_profile", ctypes.c_int, [vp, ctypes.c_int]),
+    ("mitos_ic_profile_entry", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                              ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_double)]),
     ("mitos_ic_run_reference_order", ctypes.c_int, [ctypes.POINTER(IcConfig),
                                                     ctypes.POINTER(IcResult), vp, vp, vp]),
 ]

@@ -41,13 +41,11 @@ extern "C" int mitos_pk_bin_table(int N, double boxlen, int bins, int* h) {
 
 namespace mitos {
 
-// Device copy of mitos_pk_bin_table for every integer n^2 in [1, 3(N/2)^2]:
-// u is rounded to float and multiplied by (bins-1) in float
-// (src/calc_powerspec.c:36-63).
-const int* pk_bin_lut(int N, double boxlen, int bins) {
+const int* pk_bin_edges(int N, double boxlen, int bins, int* err) {
     static std::mutex mu;
     static std::map<std::tuple<int, int, double, int>, int*> cache;
     int dev = 0;
+    *err = MITOS_ERR_DEVICE;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     auto key = std::make_tuple(dev, N, boxlen, bins);
     {
@@ -57,13 +55,32 @@ const int* pk_bin_lut(int N, double boxlen, int bins) {
     }
     const long n2max = 3L * (N / 2) * (N / 2);
     std::vector<int> h(n2max + 1);
-    mitos_pk_bin_table(N, boxlen, bins, h.data());
+    *err = mitos_pk_bin_table(N, boxlen, bins, h.data());
+    if (*err != MITOS_OK) return nullptr;
+    // k = sqrt(n^2) dk grows by a relative step >= 1/(2 n^2) between integers,
+    // far above the rounding of log, so the bin never decreases; checked here.
+    std::vector<int> first(bins + 1, (int)(n2max + 1));
+    int prev = 0;
+    for (long n2 = 1; n2 <= n2max; n2++) {
+        const int b = h[n2];
+        if (b < prev || b < 0) {
+            *err = MITOS_ERR_UNSUPPORTED;
+            return nullptr;
+        }
+        for (int q = prev + 1; q <= b; q++) first[q] = (int)n2;
+        if (n2 == 1)
+            for (int q = 0; q <= b; q++) first[q] = 1;
+        prev = b;
+    }
     int* d = nullptr;
-    if (hipMalloc(&d, sizeof(int) * h.size()) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    *err = MITOS_ERR_ALLOC;
+    if (hipMalloc(&d, sizeof(int) * first.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
         hipFree(d);
         return nullptr;
     }
+    *err = MITOS_OK;
     std::lock_guard<std::mutex> lock(mu);
     cache[key] = d;
     return d;
@@ -176,9 +193,12 @@ extern "C" int mitos_displace_tsc_dg(const struct distributed_grid* psi_x,
         return set_last_error(MITOS_ERR_REAL_SPACE);
     const int N = psi_x->N;
     if (npart == 0) return MITOS_OK;
-    hipLaunchKernelGGL((k_displace<double, false>), dim3(nblk(npart)), dim3(256), 0,
-                       api_stream(), psi_x->box, psi_y->box, psi_z->box, (long)(N + 2), N,
-                       psi_x->boxlen, 0.0f, npart, X, Y, Z);
+    // the gather indexes a grid with 32-bit element offsets
+    if ((unsigned long long)(N + 2) * N * N > 0xffffffffULL)
+        return set_last_error(MITOS_ERR_UNSUPPORTED);
+    hipLaunchKernelGGL((k_displace<double>), dim3(nblk(npart)), dim3(256), 0, api_stream(),
+                       psi_x->box, psi_y->box, psi_z->box, (unsigned)(N + 2), N, psi_x->boxlen,
+                       npart, X, Y, Z);
     MITOS_TRY(set_last_error(launch_status()));
     return set_last_error(api_sync());
 }
@@ -192,7 +212,7 @@ extern "C" int mitos_cic_overdensity(int N, double boxlen, long long npart, cons
     if (!grid) return set_last_error(MITOS_ERR_ALLOC);
     hipStream_t s = api_stream();
     MITOS_HIP_CHECK(hipMemsetAsync(grid, 0, sizeof(unsigned long long) * ncell, s));
-    hipLaunchKernelGGL(k_cic, dim3(nblk(npart)), dim3(256), 0, s, X, Y, Z, npart, N, boxlen,
+    hipLaunchKernelGGL(k_cic<double>, dim3(nblk(npart)), dim3(256), 0, s, X, Y, Z, npart, N, boxlen,
                        grid);
     MITOS_TRY(set_last_error(launch_status()));
     const double norm = ldexp(1.0, -MITOS_CIC_SHIFT) * ((double)ncell / (double)npart);

@@ -102,7 +102,7 @@ int strided_pass(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride,
     constexpr int C = ColTile<T>::C;
     constexpr int E = LineCfg<L>::E;
     constexpr int TPL = L / E;
-    constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 65536;
+    constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
     const int ncols = L / 2 + 1;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
@@ -118,19 +118,21 @@ struct RowsPerBlock {
     static constexpr int value = TPL >= 256 ? 1 : 256 / TPL;
 };
 
-template <typename T, int L, int RW, typename OP, typename RAD>
+template <typename T, int L, int RW, int HO, typename OP, typename RAD>
 struct ZC2RK;
-template <typename T, int L, int RW, typename OP, int... Rs>
-struct ZC2RK<T, L, RW, OP, Radices<Rs...>> {
+template <typename T, int L, int RW, int HO, typename OP, int... Rs>
+struct ZC2RK<T, L, RW, HO, OP, Radices<Rs...>> {
     template <typename... A>
     static void go(dim3 g, dim3 b, hipStream_t s, A... a) {
-        hipLaunchKernelGGL((k_zc2r<T, L, LineCfg<L>::E, RW, OP, Rs...>), g, b, 0, s, a...);
+        hipLaunchKernelGGL((k_zc2r<T, L, LineCfg<L>::E, RW, HO, OP, Rs...>), g, b, 0, s, a...);
     }
 };
 
-template <typename T, int N, typename OP>
+// HO > 0 writes rows with a periodic one-value halo (see k_zc2r)
+template <typename T, int N, typename OP, int HO = 0>
 int zc2r_pass(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, long outb,
               T scale, OP op, hipStream_t s) {
+    if (rs < N + (HO > 0 ? HO + 1 : 0)) return MITOS_ERR_ARG;
     constexpr int L = N / 2;
     constexpr int TPL = L / LineCfg<L>::E;
     constexpr int RW = RowsPerBlock<TPL>::value;
@@ -139,7 +141,7 @@ int zc2r_pass(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, l
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
     dim3 grid((unsigned)((nrows + RW - 1) / RW), nbatch);
-    ZC2RK<T, L, RW, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, in, out, nrows,
+    ZC2RK<T, L, RW, HO, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, in, out, nrows,
                                                      N, hs, rs, inb, outb, tw, twN, scale, op);
     return launch_status();
 }

@@ -37,6 +37,22 @@ template <> struct LineCfg<768>  { static constexpr int E = 12; typedef Radices<
 template <> struct LineCfg<1024> { static constexpr int E = 16; typedef Radices<16, 16, 4> R; };
 template <> struct LineCfg<1536> { static constexpr int E = 24; typedef Radices<8, 8, 8, 3> R; };
 
+// Line configuration with fewer values per thread, for kernels that keep
+// several transforms' worth of data live (multi-output passes, fused z pass).
+template <int L>
+struct LineCfgS;
+template <> struct LineCfgS<4>    { static constexpr int E = 4;  typedef Radices<4> R; };
+template <> struct LineCfgS<8>    { static constexpr int E = 8;  typedef Radices<8> R; };
+template <> struct LineCfgS<16>   { static constexpr int E = 4;  typedef Radices<4, 4> R; };
+template <> struct LineCfgS<32>   { static constexpr int E = 8;  typedef Radices<8, 4> R; };
+template <> struct LineCfgS<64>   { static constexpr int E = 8;  typedef Radices<8, 8> R; };
+template <> struct LineCfgS<128>  { static constexpr int E = 8;  typedef Radices<8, 8, 2> R; };
+template <> struct LineCfgS<256>  { static constexpr int E = 8;  typedef Radices<8, 8, 4> R; };
+template <> struct LineCfgS<512>  { static constexpr int E = 8;  typedef Radices<8, 8, 8> R; };
+template <> struct LineCfgS<768>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 4, 3> R; };
+template <> struct LineCfgS<1024> { static constexpr int E = 8;  typedef Radices<8, 8, 8, 2> R; };
+template <> struct LineCfgS<1536> { static constexpr int E = 12; typedef Radices<4, 4, 4, 4, 2, 3> R; };
+
 // columns per strided tile: 128-byte rows (16 float2 / 8 double2)
 template <typename T>
 struct ColTile {
@@ -74,8 +90,10 @@ __global__ void __launch_bounds__(L / E * C)
     __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
     const int c = threadIdx.x % C;
     const int t = threadIdx.x / C;
-    const int col = blockIdx.x * C + c;
-    const int o = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int col = bx * C + c;
+    const int o = by;
     const bool ok = col < ncols;
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
@@ -99,9 +117,11 @@ __global__ void __launch_bounds__(L / E * C)
 }
 
 // ---- z c2r pass ------------------------------------------------------------
-// in: rows of hs complex (first N/2+1 used); out: rows of rs reals (first N
-// written, the remaining rs-N set to zero).  L = N/2.
-template <typename T, int L, int E, int RW, typename OP, int... Rs>
+// in: rows of hs complex (first N/2+1 used); out: rows of rs reals.  HO = 0:
+// the N values at row[0..N), the remaining rs-N set to zero.  HO > 0: value z
+// at row[HO + z] with the periodic halo row[HO-1] = value N-1 and row[HO+N] =
+// value 0 (rs >= N + HO + 1; the other padding is left unwritten).  L = N/2.
+template <typename T, int L, int E, int RW, int HO, typename OP, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
     k_zc2r(const cx<T>* __restrict__ in, T* __restrict__ out, long nrows, int N, long hs,
            long rs, long in_batch, long out_batch, const cx<T>* __restrict__ tw,
@@ -158,9 +178,14 @@ __global__ void __launch_bounds__(L / E * RW)
             typename vec2<T>::type p;
             p.x = v[e].re * scale;
             p.y = v[e].im * scale;
-            *reinterpret_cast<typename vec2<T>::type*>(dst + 2 * m) = p;
+            *reinterpret_cast<typename vec2<T>::type*>(dst + HO + 2 * m) = p;
+            if (HO > 0) {
+                if (m == 0) dst[HO + 2 * L] = p.x;
+                if (m == L - 1) dst[HO - 1] = p.y;
+            }
         }
-        for (long z = 2 * L + t; z < rs; z += TPL) dst[z] = 0;
+        if (HO == 0)
+            for (long z = 2 * L + t; z < rs; z += TPL) dst[z] = 0;
     }
 }
 

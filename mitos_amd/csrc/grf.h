@@ -31,14 +31,18 @@ __device__ __forceinline__ Xo row_start(const NoiseJumps& J, long line) {
     return s;
 }
 
-// src/random.c:34-49: cosine branch of Box-Muller from two 64-bit draws
+// src/random.c:34-49: cosine branch of Box-Muller from two 64-bit draws.
+// CR: correctly rounded log/cos (bit-exact Gaussians); else the device libm
+// (within an ulp, used when the result is rounded to float anyway).
+template <bool CR = true>
 __device__ __forceinline__ double bm_normal(Xo& gen) {
     const uint64_t A = xo_next(gen);
     const uint64_t B = xo_next(gen);
     const double RMax = 18446744073709551616.0;  // (double)UINT64_MAX + 1 = 2^64
     const double u = ((double)A + 0.5) / RMax;
     const double v = ((double)B + 0.5) / RMax;
-    return sqrt(-2 * cr_log(u)) * cr_cos(2 * M_PI * v);
+    if (CR) return sqrt(-2 * cr_log(u)) * cr_cos(2 * M_PI * v);
+    return sqrt(-2 * log(u)) * cos(2 * M_PI * v);
 }
 
 // src/fft.c:29-35, evaluated without contraction so k matches the host bits
@@ -52,31 +56,54 @@ __host__ __device__ __forceinline__ void wavevec(int x, int y, int z, int N, dou
     k = sqrt(kx * kx + ky * ky + kz * kz);
 }
 
-// src/grf.c:26-69 with an amplitude amp(kx,ky,kz,k) applied to both parts.
-// One thread generates one z-row in stream order.
-template <typename T, typename AMP>
+// src/grf.c:26-69 with an amplitude amp(n2, kx, ky, kz, k) applied to both
+// parts (n2 = integer |k|^2 / dk^2).  One thread walks one z-row in stream
+// order; a block's 256 rows are staged through LDS in chunks of 16 modes so
+// the stores are 16-mode contiguous runs.
+template <typename T, typename AMP, bool CR>
 __global__ void __launch_bounds__(256)
     k_grf(cx<T>* out, long hs, int N, double boxlen, long batch_stride, NoiseJumps J,
           AMP amp) {
-    const long line = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (line >= (long)N * N) return;
+    constexpr int ZC = 16;
+    __shared__ cx<T> stage[256 * ZC];
+    const long nrows = (long)N * N;
+    const long line0 = (long)blockIdx.x * 256;
+    const long line = line0 + threadIdx.x;
+    const bool ok = line < nrows;
     const int x = (int)(line / N), y = (int)(line % N);
-    Xo gen = row_start(J, line);
+    Xo gen = ok ? row_start(J, line) : J.s0;
     const double boxvol = boxlen * boxlen * boxlen;
     const double factor = sqrt(boxvol / 2);
     const double dk = 2 * M_PI / boxlen;
-    cx<T>* row = out + blockIdx.y * batch_stride + line * hs;
-    for (int z = 0; z <= N / 2; z++) {
-        double kx, ky, kz, k;
-        wavevec(x, y, z, N, dk, kx, ky, kz, k);
-        if (k > 0) {
-            const double a = bm_normal(gen) * factor;
-            const double b = bm_normal(gen) * factor;
-            const double m = amp(kx, ky, kz, k);
-            st(row + z, mk<T>((T)(a * m), (T)(b * m)));
-        } else {
-            st(row + z, mk<T>(0, 0));
+    const int H = N / 2 + 1;
+    const int nx = freq_index(x, N), ny = freq_index(y, N);
+    const long n2xy = (long)nx * nx + (long)ny * ny;
+    cx<T>* base = out + blockIdx.y * batch_stride;
+    for (int z0 = 0; z0 < H; z0 += ZC) {
+        if (ok) {
+            for (int dz = 0; dz < ZC; dz++) {
+                const int z = z0 + dz;
+                if (z >= H) break;
+                double kx, ky, kz, k;
+                wavevec(x, y, z, N, dk, kx, ky, kz, k);
+                cx<T> val = mk<T>(0, 0);
+                if (k > 0) {
+                    const double a = bm_normal<CR>(gen) * factor;
+                    const double b = bm_normal<CR>(gen) * factor;
+                    const double m = amp(n2xy + (long)z * z, kx, ky, kz, k);
+                    val = mk<T>((T)(a * m), (T)(b * m));
+                }
+                stage[threadIdx.x * ZC + dz] = val;
+            }
         }
+        __syncthreads();
+        for (int i = 0; i < ZC; i++) {
+            const int idx = i * 256 + threadIdx.x;
+            const int r = idx / ZC, dz = idx % ZC;
+            const long l = line0 + r;
+            if (l < nrows && z0 + dz < H) st(base + l * hs + z0 + dz, stage[r * ZC + dz]);
+        }
+        __syncthreads();
     }
 }
 
@@ -105,18 +132,26 @@ __global__ void k_hermitian(cx<T>* f, long hs, int N, long batch_stride) {
 }
 
 struct AmpOne {
-    __device__ __forceinline__ double operator()(double, double, double, double) const {
+    __device__ __forceinline__ double operator()(long, double, double, double, double) const {
         return 1.0;
     }
 };
 
-template <typename T, typename AMP>
+// amplitude looked up by the integer n^2 of the mode
+struct AmpTable {
+    const double* tab;
+    __device__ __forceinline__ double operator()(long n2, double, double, double, double) const {
+        return tab[n2];
+    }
+};
+
+template <typename T, typename AMP, bool CR = true>
 int launch_grf(cx<T>* out, long hs, int N, double boxlen, int nb, long bstride,
                const NoiseJumps& J, AMP amp, hipStream_t s) {
     const long rows = (long)N * N;
     dim3 grid((unsigned)((rows + 255) / 256), nb);
-    hipLaunchKernelGGL((k_grf<T, AMP>), grid, dim3(256), 0, s, out, hs, N, boxlen, bstride, J,
-                       amp);
+    hipLaunchKernelGGL((k_grf<T, AMP, CR>), grid, dim3(256), 0, s, out, hs, N, boxlen, bstride,
+                       J, amp);
     return hipGetLastError() == hipSuccess ? MITOS_OK : MITOS_ERR_DEVICE;
 }
 

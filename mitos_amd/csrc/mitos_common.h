@@ -101,6 +101,23 @@ __device__ __forceinline__ void st(cx<T>* p, cx<T> a) {
     *reinterpret_cast<typename vec2<T>::type*>(p) = v;
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X, each with its
+// own L2.  For kernels whose neighbouring blocks share data, renumber so that
+// XCD k runs the k-th contiguous eighth of the logical blocks.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    if (nb % 8u) return b;
+    return (b % 8u) * (nb / 8u) + b / 8u;
+}
+
+// The (x, y) work-group of a 2D grid after the same remap of its linear
+// index (x fastest): neighbouring x tiles of one row run on one XCD and share
+// its L2.
+__device__ __forceinline__ void xcd_block2(unsigned& bx, unsigned& by) {
+    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    bx = lb % gridDim.x;
+    by = lb / gridDim.x;
+}
+
 // Signed frequency index of grid index i (src/fft.c:31-33: index N/2 -> +N/2).
 __host__ __device__ __forceinline__ int freq_index(int i, int N) { return (i > N / 2) ? i - N : i; }
 

@@ -6,57 +6,97 @@
 namespace mitos {
 
 // fixed-point unit of the CIC deposit: weights are summed as integers of
-// 2^-40, so the total does not depend on the order of the atomic adds
+// 2^-40 (2^-31 for float weights), so the total does not depend on the order
+// of the atomic adds
 #define MITOS_CIC_SHIFT 40
+#define MITOS_CIC_SHIFT_F32 31
 
 __host__ __device__ __forceinline__ double fwrap_d(double x, double L) {
     return fmod(fmod(x, L) + L, L);  // include/distributed_grid.h:27
 }
 
+// fwrap for |x| < L without fmod: every step of fmod(fmod(x,L)+L, L) is an
+// exact subtraction there, so the result has the same bits as fwrap_d.
+__host__ __device__ __forceinline__ double fwrap_fast(double x, double L) {
+#pragma clang fp contract(off)
+    if (!(x > -L && x < 2 * L)) return fwrap_d(x, L);
+    const double m = (x >= L) ? x - L : x;  // fmod(x, L)
+    const double a = m + L;
+    double r = (a >= L) ? a - L : a;         // fmod(a, L), a in [0, 2L]
+    if (r >= L) r -= L;
+    return r;
+}
+
 // one axis of the TSC window (src/grids_interp.c:291-296): the three taps with
 // non-zero weight, in the reference's loop order.  X is in cell units.
-__device__ __forceinline__ void tsc_axis(double X, int& i0, double w[3]) {
+template <typename M>
+__device__ __forceinline__ void tsc_axis(M X, int& i0, M w[3]) {
 #pragma clang fp contract(off)
     const int iX = (int)floor(X);
-    const double d = X - iX;
-    const int first = (d < 0.5) ? -1 : 0;  // reference loops from floor(d-1.5)
+    const M d = X - iX;
+    const int first = (d < (M)0.5) ? -1 : 0;  // reference loops from floor(d-1.5)
     i0 = iX + first;
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-        const double xx = fabs(X - (iX + first + t));
-        w[t] = xx < 0.5 ? (0.75 - xx * xx) : (xx < 1.5 ? 0.5 * (1.5 - xx) * (1.5 - xx) : 0);
+        const M xx = fabs(X - (M)(iX + first + t));
+        w[t] = xx < (M)0.5 ? ((M)0.75 - xx * xx)
+                           : (xx < (M)1.5 ? (M)0.5 * ((M)1.5 - xx) * ((M)1.5 - xx) : (M)0);
     }
 }
 
 __device__ __forceinline__ int wrapi(int i, int N) { return ((i % N) + N) % N; }
+// wrap of an index known to lie in [-N, 2N)
+__device__ __forceinline__ int wrap1(int i, int N) { return i < 0 ? i + N : (i >= N ? i - N : i); }
 
-// gridTSC_dg (src/grids_interp.c:267-309) on one rank: all taps wrap periodically.
-template <typename T>
-__device__ __forceinline__ double tsc_gather(const T* __restrict__ box, long rs, int N,
-                                             double boxlen, double x, double y, double z) {
+// periodic wrap of any int index (floor division through a float reciprocal,
+// then one correction each way; exact for |i| < 2^22)
+__device__ __forceinline__ int wrapf(int i, int N, float invN) {
+    int r = i - N * (int)floorf((float)i * invN);
+    r = r < 0 ? r + N : r;
+    return r >= N ? r - N : r;
+}
+
+// gridTSC_dg (src/grids_interp.c:267-309) on one rank: all taps wrap
+// periodically.  The 27 taps with nonzero weight are summed in the reference's
+// loop order; FMA = false keeps every product and sum separately rounded (the
+// reference's arithmetic), FMA = true fuses them (FP32 variant).  Element
+// offsets are 32-bit (rs * N * N < 2^32).
+template <typename T, typename M, bool FMA>
+__device__ __forceinline__ M tsc_gather(const T* __restrict__ box, unsigned rs, int N, float invN,
+                                        M boxlen, M x, M y, M z) {
 #pragma clang fp contract(off)
-    const double X = x * N / boxlen;
-    const double Y = y * N / boxlen;
-    const double Z = z * N / boxlen;
+    const M X = x * N / boxlen;
+    const M Y = y * N / boxlen;
+    const M Z = z * N / boxlen;
     int i0, j0, k0;
-    double wx[3], wy[3], wz[3];
+    M wx[3], wy[3], wz[3];
     tsc_axis(X, i0, wx);
     tsc_axis(Y, j0, wy);
     tsc_axis(Z, k0, wz);
-    int kk[3];
+    const int ib = wrapf(i0, N, invN), jb = wrapf(j0, N, invN), kb = wrapf(k0, N, invN);
+    unsigned ox[3], oy[3], oz[3];
+    const unsigned plane = (unsigned)N * rs;
 #pragma unroll
-    for (int c = 0; c < 3; c++) kk[c] = wrapi(k0 + c, N);
-    double sum = 0;
+    for (int t = 0; t < 3; t++) {
+        ox[t] = (unsigned)wrap1(ib + t, N) * plane;
+        oy[t] = (unsigned)wrap1(jb + t, N) * rs;
+        oz[t] = (unsigned)wrap1(kb + t, N);
+    }
+    M sum = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const long ix = wrapi(i0 + a, N);
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            const long iy = wrapi(j0 + b, N);
-            const T* row = box + (ix * N + iy) * rs;
-            const double wab = wx[a] * wy[b];
+            const unsigned row = ox[a] + oy[b];
+            const M wab = wx[a] * wy[b];
 #pragma unroll
-            for (int c = 0; c < 3; c++) sum += (double)row[kk[c]] * (wab * wz[c]);
+            for (int c = 0; c < 3; c++) {
+                const M v = (M)box[row + oz[c]];
+                if (FMA)
+                    sum = fma(v, wab * wz[c], sum);
+                else
+                    sum += v * (wab * wz[c]);
+            }
         }
     }
     return sum;
@@ -87,66 +127,440 @@ __global__ void k_lattice(int M, float spacing, long long first_id, float* X, fl
     if (ids) ids[p] = p + first_id;
 }
 
-// src/mitos.c:827-863 then :1072-1077.  LATTICE: start from the lattice
-// position (M = N) instead of reading X/Y/Z.
-template <typename T, bool LATTICE>
-__global__ void __launch_bounds__(256)
-    k_displace(const T* __restrict__ px, const T* __restrict__ py, const T* __restrict__ pz,
-               long rs, int N, double boxlen, float spacing, long long npart, float* X,
-               float* Y, float* Z) {
+// one particle: three sequential TSC displacements (src/mitos.c:827-863) then
+// the periodic wrap (:1072-1077), in the reference's double arithmetic
+template <typename T>
+__device__ __forceinline__ void displace_one(const T* __restrict__ px, const T* __restrict__ py,
+                                             const T* __restrict__ pz, unsigned rs, int N,
+                                             double boxlen, float& x, float& y, float& z) {
 #pragma clang fp contract(off)
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npart) return;
-    float x, y, z;
-    if (LATTICE) {
-        lattice_pos(p, N, spacing, x, y, z);
-    } else {
-        x = X[p];
-        y = Y[p];
-        z = Z[p];
-    }
-    double d = tsc_gather(px, rs, N, boxlen, (double)x, (double)y, (double)z);
+    const float invN = 1.0f / N;
+    double d = tsc_gather<T, double, false>(px, rs, N, invN, boxlen, x, y, z);
     x = (float)((double)x - d);
-    d = tsc_gather(py, rs, N, boxlen, (double)x, (double)y, (double)z);
+    d = tsc_gather<T, double, false>(py, rs, N, invN, boxlen, x, y, z);
     y = (float)((double)y - d);
-    d = tsc_gather(pz, rs, N, boxlen, (double)x, (double)y, (double)z);
+    d = tsc_gather<T, double, false>(pz, rs, N, invN, boxlen, x, y, z);
     z = (float)((double)z - d);
-    X[p] = (float)fwrap_d(x, boxlen);
-    Y[p] = (float)fwrap_d(y, boxlen);
-    Z[p] = (float)fwrap_d(z, boxlen);
+    x = (float)fwrap_fast(x, boxlen);
+    y = (float)fwrap_fast(y, boxlen);
+    z = (float)fwrap_fast(z, boxlen);
 }
 
-// analyse/src/mitos_read.c:281-334: eight weights per particle, summed as
-// 2^-40 fixed point into an unpadded N^3 grid of 64-bit integers.
-static __global__ void __launch_bounds__(256)
+// particles at X/Y/Z displaced through grids of row stride rs (no halo)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_displace(const T* __restrict__ px, const T* __restrict__ py, const T* __restrict__ pz,
+               unsigned rs, int N, double boxlen, long long npart, float* X, float* Y, float* Z) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npart) return;
+    float x = X[p], y = Y[p], z = Z[p];
+    displace_one<T>(px, py, pz, rs, N, boxlen, x, y, z);
+    X[p] = x;
+    Y[p] = y;
+    Z[p] = z;
+}
+
+// ---- displacement grids with a periodic z halo -------------------------------------
+// The pipeline stores each displacement grid as rows of rs = N + 8 values:
+// value (x, y, z) at row (x*N + y) offset MITOS_PSI_HO + z, with the halo
+// values z = -1 and z = N (copies of z = N-1 and z = 0) beside the row, so the
+// three z taps of a TSC stencil are always three consecutive values.
+#define MITOS_PSI_HO 4
+#define MITOS_PSI_PAD 8
+
+// FP32 TSC weights: nearest cell c = floor(X + 1/2), u = X - c in [-1/2, 1/2],
+// weights of cells c-1, c, c+1.  Same cells and (up to rounding) the same
+// weights as tsc_axis.
+__device__ __forceinline__ void tsc_axis_near(float X, int& c, float w[3]) {
+    const float cf = floorf(X + 0.5f);
+    const float u = X - cf;
+    c = (int)cf;
+    const float a = 0.5f - u, b = 0.5f + u;
+    w[0] = 0.5f * (a * a);
+    w[1] = 0.75f - u * u;
+    w[2] = 0.5f * (b * b);
+}
+
+// Byte offsets of the x and y taps c-1, c, c+1 of a wrapped centre cell
+__device__ __forceinline__ void tap_rows(int cb, int N, unsigned stride, unsigned o[3]) {
+    const int m = cb == 0 ? N - 1 : cb - 1;
+    const int p = cb == N - 1 ? 0 : cb + 1;
+    o[0] = (unsigned)m * stride;
+    o[1] = (unsigned)cb * stride;
+    o[2] = (unsigned)p * stride;
+}
+
+// TSC interpolation from a halo-padded grid.  REF = true: the weights and the
+// 27-term sum of gridTSC_dg (src/grids_interp.c:267-309) with every operation
+// rounded as there.  REF = false (float): separable weights and fused sums.
+template <typename T, typename M, bool REF>
+__device__ __forceinline__ M tsc_halo(const T* __restrict__ box, unsigned rs, int N, float invN,
+                                      M X, M Y, M Z) {
+#pragma clang fp contract(off)
+    int cx, cy, cz;
+    M wx[3], wy[3], wz[3];
+    if constexpr (REF) {
+        tsc_axis(X, cx, wx);
+        tsc_axis(Y, cy, wy);
+        tsc_axis(Z, cz, wz);
+        cx++;
+        cy++;
+        cz++;
+    } else {
+        tsc_axis_near(X, cx, wx);
+        tsc_axis_near(Y, cy, wy);
+        tsc_axis_near(Z, cz, wz);
+    }
+    const int xb = wrapf(cx, N, invN), yb = wrapf(cy, N, invN), zb = wrapf(cz, N, invN);
+    unsigned ox[3], oy[3];
+    tap_rows(xb, N, (unsigned)N * rs, ox);
+    tap_rows(yb, N, rs, oy);
+    const T* zp = box + (MITOS_PSI_HO - 1) + zb;  // value z = zb - 1 of row 0
+    M sum = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        M sa = 0;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const T* r = zp + (ox[a] + oy[b]);
+            const M v0 = (M)r[0], v1 = (M)r[1], v2 = (M)r[2];
+            if constexpr (REF) {
+                const M wab = wx[a] * wy[b];
+                sum += v0 * (wab * wz[0]);
+                sum += v1 * (wab * wz[1]);
+                sum += v2 * (wab * wz[2]);
+            } else {
+                const M sr = fmaf(wz[2], v2, fmaf(wz[1], v1, wz[0] * v0));
+                sa = fmaf(wy[b], sr, sa);
+            }
+        }
+        if constexpr (!REF) sum = fmaf(wx[a], sa, sum);
+    }
+    return sum;
+}
+
+// src/mitos.c:827-863 then :1072-1077 for lattice particles (src/particle.c:
+// 87-129) over halo-padded displacement grids.  One work-group per (lattice
+// row, chunk of NT sites along z); the FP64 instance (M = double) keeps the
+// reference's arithmetic, the FP32 one (M = float) moves particles in float.
+template <typename T, typename M, int NT>
+__global__ void __launch_bounds__(NT)
+    k_displace_lattice(const T* __restrict__ px, const T* __restrict__ py,
+                       const T* __restrict__ pz, unsigned rs, int N, int nch, double boxlen,
+                       float spacing, float cells_per_len, float* __restrict__ X,
+                       float* __restrict__ Y,
+                       float* __restrict__ Z) {
+#pragma clang fp contract(off)
+    constexpr bool REF = sizeof(M) == 8;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned row = bid / (unsigned)nch;
+    const int z = (int)(bid - row * (unsigned)nch) * NT + threadIdx.x;
+    if (z >= N) return;
+    const int qx = (int)(row / (unsigned)N), qy = (int)(row - (unsigned)qx * N);
+    float x = qx * spacing, y = qy * spacing, zz = z * spacing;  // src/particle.c:107-109
+    const float invN = 1.0f / N;
+    if constexpr (REF) {
+        const double bl = boxlen;
+        double d = tsc_halo<T, double, true>(px, rs, N, invN, (double)x * N / bl,
+                                             (double)y * N / bl, (double)zz * N / bl);
+        x = (float)((double)x - d);
+        d = tsc_halo<T, double, true>(py, rs, N, invN, (double)x * N / bl, (double)y * N / bl,
+                                      (double)zz * N / bl);
+        y = (float)((double)y - d);
+        d = tsc_halo<T, double, true>(pz, rs, N, invN, (double)x * N / bl, (double)y * N / bl,
+                                      (double)zz * N / bl);
+        zz = (float)((double)zz - d);
+    } else {
+        const float s = cells_per_len;
+        float d = tsc_halo<T, float, false>(px, rs, N, invN, x * s, y * s, zz * s);
+        x = x - d;
+        d = tsc_halo<T, float, false>(py, rs, N, invN, x * s, y * s, zz * s);
+        y = y - d;
+        d = tsc_halo<T, float, false>(pz, rs, N, invN, x * s, y * s, zz * s);
+        zz = zz - d;
+    }
+    const long long p = (long long)row * N + z;
+    X[p] = (float)fwrap_fast(x, boxlen);
+    Y[p] = (float)fwrap_fast(y, boxlen);
+    Z[p] = (float)fwrap_fast(zz, boxlen);
+}
+
+// FP32 lattice displacement.  For a coordinate on the lattice the TSC window
+// of src/grids_interp.c:291-296 has the three weights (1/8, 3/4, 1/8), so the
+// sequential moves of src/mitos.c:827-863 reduce to
+//   x -= (S_xyz psi_x)(q),  y -= TSC_x(S_yz psi_y)(x', q_y, q_z),
+//   z -= TSC_xy(S_z psi_z)(x', y', q_z),
+// with S_a the 3-tap lattice stencil along axis a.  The stencils are applied
+// in k-space when the displacement grids are made (factor 3/4 + cos(2 pi n/N)/4
+// per axis), so the moves read 1 + 3 + 9 values instead of 3 x 27.
+// Work-groups walk strips of TX x-planes row by row in y, so the x taps of
+// neighbouring rows stay in L2.
+template <int NT, int TX>
+__global__ void __launch_bounds__(NT)
+    k_displace_lattice_s(const float* __restrict__ sx, const float* __restrict__ sy,
+                         const float* __restrict__ sz, unsigned rs, int N, int nch, float Lf,
+                         float spacing, double boxlen, float* __restrict__ X,
+                         float* __restrict__ Y, float* __restrict__ Z) {
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned rowi = bid / (unsigned)nch;
+    const int qz = (int)(bid - rowi * (unsigned)nch) * NT + threadIdx.x;
+    if (qz >= N) return;
+    const unsigned xi = rowi % TX, r2 = rowi / TX;
+    const int qy = (int)(r2 % (unsigned)N);
+    const int qx = (int)(r2 / (unsigned)N) * TX + (int)xi;
+    const float invN = 1.0f / N;
+    const float Nf = (float)N;
+    const unsigned plane = (unsigned)N * rs;
+    const unsigned yz = (unsigned)qy * rs + MITOS_PSI_HO + qz;
+    float x = qx * spacing, y = qy * spacing, z = qz * spacing;  // src/particle.c:107-109
+    // x move: the stencilled x displacement at the site
+    x = x - sx[(unsigned)qx * plane + yz];
+    // y move: TSC along x at (x', q_y, q_z)
+    int cx;
+    float wx[3];
+    tsc_axis_near(x * Nf / Lf, cx, wx);
+    unsigned ox[3];
+    tap_rows(wrapf(cx, N, invN), N, plane, ox);
+    y = y - fmaf(wx[2], sy[ox[2] + yz], fmaf(wx[1], sy[ox[1] + yz], wx[0] * sy[ox[0] + yz]));
+    // z move: TSC along x and y at (x', y', q_z)
+    int cy;
+    float wy[3];
+    tsc_axis_near(y * Nf / Lf, cy, wy);
+    unsigned oy[3];
+    tap_rows(wrapf(cy, N, invN), N, rs, oy);
+    const unsigned zo = MITOS_PSI_HO + qz;
+    float d = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const unsigned o = ox[a] + zo;
+        const float r = fmaf(wy[2], sz[o + oy[2]], fmaf(wy[1], sz[o + oy[1]], wy[0] * sz[o + oy[0]]));
+        d = fmaf(wx[a], r, d);
+    }
+    z = z - d;
+    const long long p = ((long long)qx * N + qy) * N + qz;
+    X[p] = (float)fwrap_fast(x, boxlen);
+    Y[p] = (float)fwrap_fast(y, boxlen);
+    Z[p] = (float)fwrap_fast(z, boxlen);
+}
+
+// analyse/src/mitos_read.c:285-320: base cell and the eight CIC weights of a
+// particle, the weights as 2^-40 fixed point in the reference's corner order
+// (000, 100, 010, 001, 110, 101, 011, 111).
+__device__ __forceinline__ void cic_cell(float X, float Y, float Z, int N, double boxlen, int& iX,
+                                         int& iY, int& iZ, double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+    const double cell = boxlen / N;
+    const double Xc = fwrap_fast((double)X, boxlen) / cell;
+    const double Yc = fwrap_fast((double)Y, boxlen) / cell;
+    const double Zc = fwrap_fast((double)Z, boxlen) / cell;
+    iX = (int)floor(Xc);
+    iY = (int)floor(Yc);
+    iZ = (int)floor(Zc);
+    dx = Xc - iX;
+    dy = Yc - iY;
+    dz = Zc - iZ;
+}
+
+__device__ __forceinline__ void cic_weights(double dx, double dy, double dz,
+                                            unsigned long long w[8]) {
+#pragma clang fp contract(off)
+    const double tx = 1.0 - dx, ty = 1.0 - dy, tz = 1.0 - dz;
+    const double S = (double)(1ULL << MITOS_CIC_SHIFT);
+    w[0] = (unsigned long long)(tx * ty * tz * S + 0.5);
+    w[1] = (unsigned long long)(dx * ty * tz * S + 0.5);
+    w[2] = (unsigned long long)(tx * dy * tz * S + 0.5);
+    w[3] = (unsigned long long)(tx * ty * dz * S + 0.5);
+    w[4] = (unsigned long long)(dx * dy * tz * S + 0.5);
+    w[5] = (unsigned long long)(dx * ty * dz * S + 0.5);
+    w[6] = (unsigned long long)(tx * dy * dz * S + 0.5);
+    w[7] = (unsigned long long)(dx * dy * dz * S + 0.5);
+}
+
+// Float variant of cic_cell/cic_weights (FP32 pipeline): the offset in the
+// cell is formed as X - iX * cell with one rounding, so it keeps the float
+// position's accuracy; weights in units of 2^-31 (a weight of 1 still fits).
+__device__ __forceinline__ void cic_cell_f(float X, float Y, float Z, int N, float cell,
+                                           float cinv, double boxlen, int& iX, int& iY, int& iZ,
+                                           float& dx, float& dy, float& dz) {
+    const float P[3] = {(float)fwrap_fast(X, boxlen), (float)fwrap_fast(Y, boxlen),
+                        (float)fwrap_fast(Z, boxlen)};
+    int I[3];
+    float D[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int i = (int)floorf(P[a] * cinv);
+        float r = fmaf(-(float)i, cell, P[a]);
+        if (r < 0.0f) {
+            i--;
+            r += cell;
+        } else if (r >= cell) {
+            i++;
+            r -= cell;
+        }
+        I[a] = i;
+        D[a] = fminf(r * cinv, 1.0f);
+    }
+    iX = I[0];
+    iY = I[1];
+    iZ = I[2];
+    dx = D[0];
+    dy = D[1];
+    dz = D[2];
+}
+
+__device__ __forceinline__ void cic_weights(float dx, float dy, float dz,
+                                            unsigned long long w[8]) {
+    const float tx = 1.0f - dx, ty = 1.0f - dy, tz = 1.0f - dz;
+    const float S = (float)(1u << MITOS_CIC_SHIFT_F32);
+    const float a = tx * ty, b = dx * ty, c = tx * dy, d = dx * dy;
+    w[0] = (unsigned)fmaf(a * tz, S, 0.5f);
+    w[1] = (unsigned)fmaf(b * tz, S, 0.5f);
+    w[2] = (unsigned)fmaf(c * tz, S, 0.5f);
+    w[3] = (unsigned)fmaf(a * dz, S, 0.5f);
+    w[4] = (unsigned)fmaf(d * tz, S, 0.5f);
+    w[5] = (unsigned)fmaf(b * dz, S, 0.5f);
+    w[6] = (unsigned)fmaf(c * dz, S, 0.5f);
+    w[7] = (unsigned)fmaf(d * dz, S, 0.5f);
+}
+
+// eight global atomic adds at wrapped base cell (iX, iY, iZ)
+__device__ __forceinline__ void cic_global(unsigned long long* grid, int N, int iX, int iY, int iZ,
+                                           const unsigned long long w[8]) {
+    const long x0 = wrapi(iX, N), x1 = wrap1((int)x0 + 1, N);
+    const long y0 = wrapi(iY, N), y1 = wrap1((int)y0 + 1, N);
+    const long z0 = wrapi(iZ, N), z1 = wrap1((int)z0 + 1, N);
+    atomicAdd(grid + (x0 * N + y0) * N + z0, w[0]);
+    atomicAdd(grid + (x1 * N + y0) * N + z0, w[1]);
+    atomicAdd(grid + (x0 * N + y1) * N + z0, w[2]);
+    atomicAdd(grid + (x0 * N + y0) * N + z1, w[3]);
+    atomicAdd(grid + (x1 * N + y1) * N + z0, w[4]);
+    atomicAdd(grid + (x1 * N + y0) * N + z1, w[5]);
+    atomicAdd(grid + (x0 * N + y1) * N + z1, w[6]);
+    atomicAdd(grid + (x1 * N + y1) * N + z1, w[7]);
+}
+
+// analyse/src/mitos_read.c:281-334 one thread per particle, global atomics
+// (any particle order; used when the lattice tiling below does not apply)
+// (W as in k_cic_lattice)
+template <typename W = double>
+__global__ void __launch_bounds__(256)
     k_cic(const float* __restrict__ X, const float* __restrict__ Y, const float* __restrict__ Z,
           long long npart, int N, double boxlen, unsigned long long* grid) {
-#pragma clang fp contract(off)
     const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= npart) return;
-    const double cell = boxlen / N;
-    const double Xc = fwrap_d((double)X[l], boxlen) / cell;
-    const double Yc = fwrap_d((double)Y[l], boxlen) / cell;
-    const double Zc = fwrap_d((double)Z[l], boxlen) / cell;
-    const int iX = (int)floor(Xc), iY = (int)floor(Yc), iZ = (int)floor(Zc);
-    const double dx = Xc - iX, dy = Yc - iY, dz = Zc - iZ;
-    const double tx = 1.0 - dx, ty = 1.0 - dy, tz = 1.0 - dz;
-    const long x0 = wrapi(iX, N), x1 = wrapi(iX + 1, N);
-    const long y0 = wrapi(iY, N), y1 = wrapi(iY + 1, N);
-    const long z0 = wrapi(iZ, N), z1 = wrapi(iZ + 1, N);
-    const double S = (double)(1ULL << MITOS_CIC_SHIFT);
-#define MITOS_DEP(a, b, c, w) \
-    atomicAdd(grid + ((a) * N + (b)) * N + (c), (unsigned long long)((w) * S + 0.5))
-    MITOS_DEP(x0, y0, z0, tx * ty * tz);
-    MITOS_DEP(x1, y0, z0, dx * ty * tz);
-    MITOS_DEP(x0, y1, z0, tx * dy * tz);
-    MITOS_DEP(x0, y0, z1, tx * ty * dz);
-    MITOS_DEP(x1, y1, z0, dx * dy * tz);
-    MITOS_DEP(x1, y0, z1, dx * ty * dz);
-    MITOS_DEP(x0, y1, z1, tx * dy * dz);
-    MITOS_DEP(x1, y1, z1, dx * dy * dz);
-#undef MITOS_DEP
+    int iX, iY, iZ;
+    W dx, dy, dz;
+    if constexpr (sizeof(W) == 8)
+        cic_cell(X[l], Y[l], Z[l], N, boxlen, iX, iY, iZ, dx, dy, dz);
+    else
+        cic_cell_f(X[l], Y[l], Z[l], N, (float)(boxlen / N), (float)(N / boxlen), boxlen, iX, iY,
+                   iZ, dx, dy, dz);
+    unsigned long long w[8];
+    cic_weights(dx, dy, dz, w);
+    cic_global(grid, N, iX, iY, iZ, w);
 }
+
+// CIC deposit (analyse/src/mitos_read.c:281-334) of particles stored in
+// lattice order (X[(qx*N + qy)*N + qz] belongs to lattice site q).  A
+// work-group owns a PX*PY*PZ tile of lattice sites and deposits them into an
+// LDS window of RX*RY*RZ cells placed at the tile's mean displacement.  The
+// window sums integers, so the order of the LDS and global atomics does not
+// change the result; particles whose cloud leaves the window go straight to
+// the grid, and the window's non-zero cells are added to the grid at the end.
+// W = double: the reference's double weights in units of 2^-40 (FP64
+// pipeline); W = float: float weights in units of 2^-31 (FP32 pipeline).
+template <typename W, int PX, int PY, int PZ, int RX, int RY, int RZ, int NT>
+__global__ void __launch_bounds__(NT)
+    k_cic_lattice(const float* __restrict__ X, const float* __restrict__ Y,
+                  const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid) {
+    constexpr int P = PX * PY * PZ, K = P / NT, R = RX * RY * RZ;
+    static_assert(P % NT == 0, "tile must be a multiple of the work-group");
+    __shared__ unsigned long long win[R];
+    __shared__ int red[3][NT / 64];
+    const int ntz = N / PZ, nty = N / PY;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = bid % ntz, ty = (bid / ntz) % nty, tx = bid / (ntz * nty);
+    const float cell = (float)(boxlen / N), cinv = (float)(N / boxlen);
+    int bx[K], by[K], bz[K];
+    W fx[K], fy[K], fz[K];
+    int sx = 0, sy = 0, sz = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        const int p = threadIdx.x + NT * i;
+        const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+        const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
+        const long long idx = ((long long)qx * N + qy) * N + qz;
+        int iX, iY, iZ;
+        if constexpr (sizeof(W) == 8)
+            cic_cell(X[idx], Y[idx], Z[idx], N, boxlen, iX, iY, iZ, fx[i], fy[i], fz[i]);
+        else
+            cic_cell_f(X[idx], Y[idx], Z[idx], N, cell, cinv, boxlen, iX, iY, iZ, fx[i], fy[i],
+                       fz[i]);
+        int rx = iX - qx, ry = iY - qy, rz = iZ - qz;
+        rx = rx < -N / 2 ? rx + N : (rx >= N / 2 ? rx - N : rx);
+        ry = ry < -N / 2 ? ry + N : (ry >= N / 2 ? ry - N : ry);
+        rz = rz < -N / 2 ? rz + N : (rz >= N / 2 ? rz - N : rz);
+        bx[i] = qx + rx;
+        by[i] = qy + ry;
+        bz[i] = qz + rz;
+        sx += rx;
+        sy += ry;
+        sz += rz;
+    }
+    for (int c = threadIdx.x; c < R; c += NT) win[c] = 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+        sz += __shfl_xor(sz, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sx;
+        red[1][threadIdx.x >> 6] = sy;
+        red[2][threadIdx.x >> 6] = sz;
+    }
+    __syncthreads();
+    int mx = 0, my = 0, mz = 0;
+    for (int w = 0; w < NT / 64; w++) {
+        mx += red[0][w];
+        my += red[1][w];
+        mz += red[2][w];
+    }
+    const int ox = tx * PX + (int)rintf((float)mx / P) - (RX - PX - 1) / 2;
+    const int oy = ty * PY + (int)rintf((float)my / P) - (RY - PY - 1) / 2;
+    const int oz = tz * PZ + (int)rintf((float)mz / P) - (RZ - PZ - 1) / 2;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        unsigned long long w[8];
+        cic_weights(fx[i], fy[i], fz[i], w);
+        const int ax = bx[i] - ox, ay = by[i] - oy, az = bz[i] - oz;
+        if (ax >= 0 && ax + 1 < RX && ay >= 0 && ay + 1 < RY && az >= 0 && az + 1 < RZ) {
+            unsigned long long* b = win + (ax * RY + ay) * RZ + az;
+            atomicAdd(b, w[0]);
+            atomicAdd(b + RY * RZ, w[1]);
+            atomicAdd(b + RZ, w[2]);
+            atomicAdd(b + 1, w[3]);
+            atomicAdd(b + RY * RZ + RZ, w[4]);
+            atomicAdd(b + RY * RZ + 1, w[5]);
+            atomicAdd(b + RZ + 1, w[6]);
+            atomicAdd(b + RY * RZ + RZ + 1, w[7]);
+        } else {
+            cic_global(grid, N, bx[i], by[i], bz[i], w);
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < R; c += NT) {
+        const unsigned long long v = win[c];
+        if (v) {
+            const int cz = c % RZ, cy = (c / RZ) % RY, cx = c / (RZ * RY);
+            const long gx = wrapi(ox + cx, N), gy = wrapi(oy + cy, N), gz = wrapi(oz + cz, N);
+            atomicAdd(grid + (gx * N + gy) * N + gz, v);
+        }
+    }
+}
+
+// tile of the lattice-ordered deposit kernel (N must be a multiple of 32)
+#define MITOS_CIC_TILE 8, 16, 32, 16, 24, 40, 512
 
 // analyse/src/mitos_read.c:363-376 for equal masses: delta = sum_w * N^3/npart - 1
 template <typename T>

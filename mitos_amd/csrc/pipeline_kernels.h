@@ -22,11 +22,13 @@ namespace mitos {
 
 // ---- multi-output strided pass ------------------------------------------------
 // Output j = FFT_DIR( m_j(k_axis) * in[src_j] ) where the multiplier code is
-// 0: 1, 1: i k, 2: -k^2 (k = signed frequency of the line position times dk).
+// 0: 1, 1: i k, 2: -k^2, 3: i k S (k = signed frequency of the line position
+// times dk, S = smooth[line position]).
 template <typename T, int NIN, int NOUT>
 struct MultiPtrs {
     const cx<T>* in[NIN];
     cx<T>* out[NOUT];
+    const T* smooth = nullptr;  // per line position, for multiplier code 3
 };
 
 template <int... CODES>
@@ -40,7 +42,7 @@ struct OutCodes {
 
 template <typename T>
 __device__ __forceinline__ cx<T> apply_mul(int code, cx<T> v, T k) {
-    if (code == 1) return mk<T>(-v.im * k, v.re * k);  // i k v
+    if (code == 1 || code == 3) return mk<T>(-v.im * k, v.re * k);  // i k v
     if (code == 2) {
         const T m = -(k * k);
         return mk<T>(v.re * m, v.im * m);
@@ -59,16 +61,15 @@ __global__ void __launch_bounds__(L / E * C)
     __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
     const int c = threadIdx.x % C;
     const int t = threadIdx.x / C;
-    const int col = blockIdx.x * C + c;
-    const int o = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int col = bx * C + c;
+    const int o = by;
     const bool ok = col < ncols;
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = (long)o * ostride + col;
     cx<T> v[E], u[E];
-    T kax[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) kax[e] = (T)(freq_index(t + TPL * e, L) * dk);
 #pragma unroll
     for (int j = 0; j < NOUT; j++) {
         if (j == 0 || CODES::src(j) != CODES::src(j > 0 ? j - 1 : 0)) {
@@ -78,7 +79,12 @@ __global__ void __launch_bounds__(L / E * C)
                 v[e] = ok ? ld(in + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
         }
 #pragma unroll
-        for (int e = 0; e < E; e++) u[e] = apply_mul(CODES::mul(j), v[e], kax[e]);
+        for (int e = 0; e < E; e++) {
+            const int pos = t + TPL * e;
+            T k = (T)(freq_index(pos, L) * dk);
+            if (CODES::mul(j) == 3) k *= P.smooth[pos];
+            u[e] = apply_mul(CODES::mul(j), v[e], k);
+        }
         line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
         if (ok) {
             cx<T>* out = P.out[j];
@@ -108,18 +114,18 @@ __global__ void __launch_bounds__(L / E * RW)
     const bool ok = row < nrows;
     cx<T> xx[E], yy[E], s[E], cur[E];
     // component order: xx, yy, zz, xy, yz, xz
-    const int srcs[6] = {5, 2, 0, 4, 1, 3};
-    const int muls[6] = {0, 0, 2, 0, 1, 1};
 #pragma unroll
     for (int q = 0; q < 6; q++) {
+        const int src_q = (q == 0) ? 5 : (q == 1) ? 2 : (q == 2) ? 0 : (q == 3) ? 4 : (q == 4) ? 1 : 3;
+        const int mul_q = (q == 2) ? 2 : (q >= 4) ? 1 : 0;
         // stage the row, with its k_z multiplier, in LDS
         if (ok) {
-            const cx<T>* src = P.in[srcs[q]] + row * hs;
+            const cx<T>* src = P.in[src_q] + row * hs;
 #pragma unroll
             for (int e = 0; e <= E; e++) {
                 const int k = (e < E) ? t + TPL * e : L;
                 if (e == E && t != 0) break;
-                cx<T> a = apply_mul(muls[q], ld(src + k), (T)(k * dk));
+                cx<T> a = apply_mul(mul_q, ld(src + k), (T)(k * dk));
                 lre[M::idx(k, rl)] = a.re;
                 lim[M::idx(k, rl)] = a.im;
             }
@@ -193,8 +199,10 @@ struct CombineParams {
     double dk;
     double sD;      // forward normalisation L^3/N^3 of delta2
     double f1, f2;  // 2LPT factors
-    double wfac;    // 0.5 * boxlen / N, argument scale of the TSC window
-    int window;     // window order (3 = TSC), 0 = none
+    const double* win;  // 1/sinc^3 per axis (TSC window), or nullptr
+    // lattice TSC stencil per axis, S[n] = 3/4 + cos(2 pi n/N)/4, or nullptr:
+    // C0 (the y and z displacements) takes S(kz), C1 (x) S(kx) S(ky) S(kz)
+    const double* smooth;
 };
 
 template <typename T, int L, int E, int C, bool SPLIT, int... Rs>
@@ -207,8 +215,10 @@ __global__ void __launch_bounds__(L / E * C)
     __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
     const int c = threadIdx.x % C;
     const int t = threadIdx.x / C;
-    const int col = blockIdx.x * C + c;
-    const int y = blockIdx.y;
+    unsigned bx, by;
+    xcd_block2(bx, by);
+    const int col = bx * C + c;
+    const int y = by;
     const bool ok = col < ncols;
     const long pstride = (long)L * hs;
     const long base = (long)y * hs + col;
@@ -218,35 +228,40 @@ __global__ void __launch_bounds__(L / E * C)
         v[e] = ok ? ld(D + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
     line_fft<T, L, E, C, true, -1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
     const double ky = freq_index(y, L) * cp.dk;
-    const double kz = col * cp.dk;
-    T kxs[E];
+    const double kz = (ok ? col : 0) * cp.dk;
+    const double wyz = cp.win ? cp.win[y] * cp.win[ok ? col : 0] : 1.0;
+    const double kyz2 = ky * ky + kz * kz;
+    cx<T> u[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int x = t + TPL * e;
         const double kx = freq_index(x, L) * cp.dk;
-        kxs[e] = (T)kx;
-        const double k = sqrt(kx * kx + ky * ky + kz * kz);
+        const double k2 = kx * kx + kyz2;
         const cx<T> phi1 = ok ? ld(K + base + (long)x * pstride) : mk<T>(0, 0);
-        // phi2 = inv_poisson(delta2(k))  (include/fft_kernels.h:96-99)
-        const double ip = (k > 0) ? -1.0 / k / k : 1.0;
-        const double d2re = (double)v[e].re * cp.sD, d2im = (double)v[e].im * cp.sD;
-        double pre = (double)phi1.re * cp.f1 + cp.f2 * (d2re * ip);
-        double pim = (double)phi1.im * cp.f1 + cp.f2 * (d2im * ip);
-        if (cp.window && k > 0) {
-            // include/fft_kernels.h:198-219
-            const double W_x = sinc_d(kx * cp.wfac);
-            const double W_y = sinc_d(ky * cp.wfac);
-            const double W_z = sinc_d(kz * cp.wfac);
-            const double W = pow(W_x * W_y * W_z, (double)cp.window);
-            const double iw = 1.0 / W;
+        // phi2 = inv_poisson(delta2(k)) (include/fft_kernels.h:96-99), DC x 1
+        const double ip = (k2 > 0) ? -1.0 / k2 : 1.0;
+        double pre = (double)phi1.re * cp.f1 + cp.f2 * ((double)v[e].re * cp.sD * ip);
+        double pim = (double)phi1.im * cp.f1 + cp.f2 * ((double)v[e].im * cp.sD * ip);
+        if (cp.win && k2 > 0) {
+            const double iw = cp.win[x] * wyz;  // 1/W_TSC(k), include/fft_kernels.h:198-219
             pre *= iw;
             pim *= iw;
         }
+        if (cp.smooth) {
+            const double sz = cp.smooth[ok ? col : 0];
+            pre *= sz;
+            pim *= sz;
+            // x displacement: i kx S(kx) S(ky) times the value
+            const double kxs = kx * (cp.smooth[x] * cp.smooth[y]);
+            u[e] = mk<T>((T)(-pim * kxs), (T)(pre * kxs));
+        }
         v[e] = mk<T>((T)pre, (T)pim);
     }
-    cx<T> u[E];
+    if (!cp.smooth) {
 #pragma unroll
-    for (int e = 0; e < E; e++) u[e] = apply_mul(1, v[e], kxs[e]);
+        for (int e = 0; e < E; e++)
+            u[e] = apply_mul(1, v[e], (T)(freq_index(t + TPL * e, L) * cp.dk));
+    }
     line_fft<T, L, E, C, true, 1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
     if (ok) {
 #pragma unroll
@@ -286,24 +301,14 @@ struct CicLoader {
 };
 
 // P(k) value transform: forward normalisation and the CIC window (order 2)
+// from the separable table (analyse/src/mitos_read.c:414-417)
 struct PkCicTransform {
-    double s;     // L^3/N^3
-    double wfac;  // 0.5 * boxlen / N
-    template <typename T>
-    __device__ __forceinline__ cx<double> operator()(cx<T> v, double kx, double ky, double kz,
-                                                     double k) const {
-#pragma clang fp contract(off)
-        double re = (double)v.re * s, im = (double)v.im * s;
-        if (k != 0) {
-            const double W_x = sinc_d(kx * wfac);
-            const double W_y = sinc_d(ky * wfac);
-            const double W_z = sinc_d(kz * wfac);
-            const double W = pow(W_x * W_y * W_z, 2.0);
-            re *= 1.0 / W;
-            im *= 1.0 / W;
-        }
-        return mk<double>(re, im);
-    }
+    double s;            // L^3/N^3
+    const double* win;   // 1/sinc^2 per axis
+    int N;
+    // factor of a row (x, y): s * w[x] * w[y]; the mode value is v * row * w[z]
+    __device__ __forceinline__ double row(int x, int y) const { return s * win[x] * win[y]; }
+    __device__ __forceinline__ double col(int z) const { return win[z]; }
 };
 
 }  // namespace mitos

@@ -32,68 +32,88 @@ __device__ __forceinline__ long long wave_sum_i(long long v) {
     return v;
 }
 
+// value transform of the binned modes: mode (x, y, z) is multiplied by
+// row(x, y) * col(z) (a separable real factor)
 struct PkNoTransform {
-    template <typename T>
-    __device__ __forceinline__ cx<double> operator()(cx<T> v, double, double, double,
-                                                     double) const {
-        return mk<double>((double)v.re, (double)v.im);
-    }
+    __device__ __forceinline__ double row(int, int) const { return 1.0; }
+    __device__ __forceinline__ double col(int) const { return 1.0; }
 };
+
+// Bin of a mode with integer n^2 = nx^2 + ny^2 + nz^2 > 0: the reference's
+// bin (src/calc_powerspec.c:58-59) is nondecreasing in n^2, so it is the
+// largest b with first[b] <= n^2.  A float estimate of the log-spaced bin is
+// corrected against the table (usually no step).
+__device__ __forceinline__ int pk_bin_of(long n2, const int* first, int bins, float cinv) {
+    int b = (int)(__logf((float)n2) * cinv);
+    b = b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
+    while (b < bins - 1 && n2 >= first[b + 1]) b++;
+    while (b > 0 && n2 < first[b]) b--;
+    return b;
+}
 
 // Partials layout: part[(blk*bins + b)*4 + {sumP, compP, sumK, compK}], cnt[blk*bins + b]
 template <typename T, typename XF>
 __global__ void __launch_bounds__(256)
     k_pk_partial(const cx<T>* __restrict__ a, const cx<T>* __restrict__ b, int N, long hs,
-                 double dk, const int* __restrict__ lut, int bins, double* part,
+                 const int* __restrict__ edges, float cinv, double dk, int bins, double* part,
                  long long* cnt, XF xf) {
 #pragma clang fp contract(off)
-    extern __shared__ double sm[];  // [4 waves][bins][4] doubles + [4][bins] counts
+    // [4 waves][bins][4] doubles, [4][bins] counts, [bins + 1] bin edges
+    extern __shared__ double sm[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     double* acc = sm + (size_t)wv * bins * 4;
     long long* cacc = reinterpret_cast<long long*>(sm + (size_t)4 * bins * 4) + (size_t)wv * bins;
+    int* first = reinterpret_cast<int*>(sm + (size_t)4 * bins * 4 + (size_t)4 * bins);
     for (int i = lane; i < bins * 4; i += 64) acc[i] = 0;
     for (int i = lane; i < bins; i += 64) cacc[i] = 0;
+    for (int i = threadIdx.x; i <= bins; i += blockDim.x) first[i] = edges[i];
     __syncthreads();
-    const long H = N / 2 + 1;
-    const long nmodes = (long)N * N * H;
-    const long stride = (long)gridDim.x * blockDim.x;
-    for (long base = (long)blockIdx.x * blockDim.x + wv * 64; base < nmodes; base += stride) {
-        const long i = base + lane;
-        int mybin = -1;
-        double vP = 0, vK = 0;
-        int mult = 0;
-        if (i < nmodes) {
-            const int z = (int)(i % H);
-            const long row = i / H;
-            const int y = (int)(row % N), x = (int)(row / N);
-            const int nx = freq_index(x, N), ny = freq_index(y, N);
-            const long n2 = (long)nx * nx + (long)ny * ny + (long)z * z;
-            if (n2 > 0) {
-                double kx, ky, kz, k;
-                wavevec(x, y, z, N, dk, kx, ky, kz, k);
-                const cx<double> u = xf(a[row * hs + z], kx, ky, kz, k);
-                const cx<double> v = xf(b[row * hs + z], kx, ky, kz, k);
-                mybin = lut[n2];
-                mult = (z == 0 || z == N / 2) ? 1 : 2;
-                vP = mult * (u.re * v.re + u.im * v.im);
-                vK = mult * k;
+    const int H = N / 2 + 1;
+    const long nrows = (long)N * N;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    // one wavefront per z-row at a time, 64 consecutive modes per step
+    for (long row = (long)blockIdx.x * (blockDim.x >> 6) + wv; row < nrows; row += nwaves) {
+        const int x = (int)(row / N), y = (int)(row % N);
+        const int nx = freq_index(x, N), ny = freq_index(y, N);
+        const long n2xy = (long)nx * nx + (long)ny * ny;
+        const double fr = xf.row(x, y);
+        const cx<T>* ra = a + row * hs;
+        const cx<T>* rb = b + row * hs;
+        for (int z0 = 0; z0 < H; z0 += 64) {
+            const int z = z0 + lane;
+            int mybin = -1;
+            double vP = 0, vK = 0;
+            const bool edge = (z == 0 || z == N / 2);
+            if (z < H) {
+                const long n2 = n2xy + (long)z * z;
+                if (n2 > 0) {
+                    const double f = fr * xf.col(z);
+                    const cx<T> u = ra[z], v = rb[z];
+                    const double mult = edge ? 1.0 : 2.0;
+                    mybin = pk_bin_of(n2, first, bins, cinv);
+                    // (f u).(f v) with the factor applied to each operand
+                    vP = mult * (((double)u.re * f) * ((double)v.re * f) +
+                                 ((double)u.im * f) * ((double)v.im * f));
+                    vK = mult * (sqrt((double)n2) * dk);
+                }
             }
-        }
-        unsigned long long act = __ballot(mybin >= 0);
-        while (act) {
-            const int lead = __ffsll((long long)act) - 1;
-            const int bl = __shfl(mybin, lead, 64);
-            const bool mine = (mybin == bl);
-            const double sP = wave_sum(mine ? vP : 0.0);
-            const double sK = wave_sum(mine ? vK : 0.0);
-            const long long sC = wave_sum_i(mine ? (long long)mult : 0LL);
-            if (lane == 0) {
-                kadd(acc[bl * 4 + 0], acc[bl * 4 + 1], sP);
-                kadd(acc[bl * 4 + 2], acc[bl * 4 + 3], sK);
-                cacc[bl] += sC;
+            unsigned long long act = __ballot(mybin >= 0);
+            while (act) {
+                const int lead = __ffsll((long long)act) - 1;
+                const int bl = __shfl(mybin, lead, 64);
+                const bool mine = (mybin == bl);
+                const unsigned long long m = __ballot(mine);
+                const unsigned long long me = __ballot(mine && edge);
+                const double sP = wave_sum(mine ? vP : 0.0);
+                const double sK = wave_sum(mine ? vK : 0.0);
+                if (lane == 0) {
+                    kadd(acc[bl * 4 + 0], acc[bl * 4 + 1], sP);
+                    kadd(acc[bl * 4 + 2], acc[bl * 4 + 3], sK);
+                    cacc[bl] += 2LL * __popcll(m) - __popcll(me);
+                }
+                act &= ~m;
             }
-            act &= ~__ballot(mine);
         }
     }
     __syncthreads();
@@ -138,12 +158,18 @@ static __global__ void k_pk_final(const double* part, const long long* cnt, int 
     obs[bn] = c;
 }
 
-// Host: device LUT of the reference bin index per integer n^2 = nx^2+ny^2+nz^2.
-const int* pk_bin_lut(int N, double boxlen, int bins);
+// Host: device table first[b], b in [0, bins], of the smallest n^2 in
+// [1, 3(N/2)^2] whose reference bin is >= b (first[bins] = 3(N/2)^2 + 1).
+// Returns nullptr with *err set when the table cannot be made.
+const int* pk_bin_edges(int N, double boxlen, int bins, int* err);
+// estimate scale of pk_bin_of: (bins-1) / (2 log(sqrt(3) N/2))
+inline float pk_bin_cinv(int N, int bins) {
+    return (float)((bins - 1) / (2.0 * log(sqrt(3.0) * N / 2)));
+}
 // Host: number of work-groups used by k_pk_partial
 inline int pk_blocks(long nmodes) {
     long b = (nmodes + 255) / 256;
-    return (int)(b < 1024 ? b : 1024);
+    return (int)(b < 2048 ? b : 2048);
 }
 
 // Binned P(k) of two half-complex grids with row stride hs; host outputs
@@ -151,8 +177,9 @@ inline int pk_blocks(long nmodes) {
 template <typename T, typename XF>
 inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen, int bins, XF xf,
            double* sums, long long* obs, hipStream_t s, double* dev_sums, long long* dev_obs) {
-    const int* lut = pk_bin_lut(N, boxlen, bins);
-    if (!lut) return MITOS_ERR_ALLOC;
+    int err = MITOS_OK;
+    const int* edges = pk_bin_edges(N, boxlen, bins, &err);
+    if (!edges) return err;
     const long nmodes = (long)N * N * (N / 2 + 1);
     const int nblk = pk_blocks(nmodes);
     const size_t pbytes = sizeof(double) * 4 * bins * nblk;
@@ -164,10 +191,10 @@ inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen,
     long long* cnt = (long long*)(ws + pbytes);
     double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes + cbytes);
     long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + cbytes + sizeof(double) * 2 * bins);
-    const size_t shm = sizeof(double) * 4 * 4 * bins + sizeof(long long) * 4 * bins;
-    const double dk = 2 * M_PI / boxlen;
-    hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, dk,
-                       lut, bins, part, cnt, xf);
+    const size_t shm =
+        sizeof(double) * 4 * 4 * bins + sizeof(long long) * 4 * bins + sizeof(int) * (bins + 1);
+    hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, edges,
+                       pk_bin_cinv(N, bins), 2 * M_PI / boxlen, bins, part, cnt, xf);
     MITOS_TRY(launch_status());
     hipLaunchKernelGGL(k_pk_final, dim3((bins + 63) / 64), dim3(64), 0, s, part, cnt, nblk,
                        bins, fs, fo);

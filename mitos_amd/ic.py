@@ -211,6 +211,22 @@ class IC:
         cfg = cfg or self.cfg
         check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
 
+    def profile(self, enable: bool = True) -> None:
+        check(lib().mitos_ic_profile(self.h, int(enable)), "mitos_ic_profile")
+
+    def profile_entries(self) -> list:
+        """[(kernel, ms, algorithmic bytes)] of the last profiled run."""
+        name = ctypes.c_char_p()
+        ms = ctypes.c_float()
+        by = ctypes.c_double()
+        n = lib().mitos_ic_profile_entry(self.h, -1, None, None, None)
+        out = []
+        for i in range(n):
+            lib().mitos_ic_profile_entry(self.h, i, ctypes.byref(name), ctypes.byref(ms),
+                                         ctypes.byref(by))
+            out.append((name.value.decode(), ms.value, by.value))
+        return out
+
     def positions(self):
         import torch
         n = self.cfg.N ** 3

@@ -126,10 +126,14 @@ def test_grf_stream_and_hermiticity(gpu, oracle):
         # state advanced by exactly the draws the reference consumes
         nd = 4 * (N * N * (N // 2 + 1) - 1)
         assert list(st.s) == [int(v) for v in _state_after(oracle, seed, nd)]
-        same = np.mean(got.view(np.uint64) == ref.view(np.uint64))
+        eq = got.view(np.uint64) == ref.view(np.uint64)
+        same = np.mean(eq)
         print("  bit-identical fraction", same, flush=True)
         assert rel(got, ref) < 1e-15
         assert same > 0.99
+        # every value that differs from glibc equals the correctly rounded
+        # evaluation of src/random.c:34-49 (glibc's log/cos misround there)
+        _check_mismatches_correctly_rounded(got, ref, eq, N, L, seed)
         assert gpu.enforce_hermiticity(g) == 0
         refh = oracle.grf(N, L, seed, hermitian=True)
         goth = g.fbox()
@@ -139,6 +143,45 @@ def test_grf_stream_and_hermiticity(gpu, oracle):
             M = np.conj(P[(-np.arange(N)) % N][:, (-np.arange(N)) % N])
             assert np.array_equal(P, M)
         g.free()
+
+
+def _cr_normal(A, B):
+    """sqrt(-2 log u) cos(2 pi v) with log and cos correctly rounded (60 digits)."""
+    import math
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    u = (float(A) + 0.5) / 18446744073709551616.0
+    v = (float(B) + 0.5) / 18446744073709551616.0
+    lg = float(Decimal(u).ln())
+    t = 2 * math.pi * v
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
+    x = Decimal(t)
+    k = int((x / (pi / 2)).to_integral_value())
+    r = x - k * (pi / 2)
+    c = s = Decimal(0)
+    term_c, term_s = Decimal(1), r
+    for i in range(0, 80, 2):
+        c += term_c
+        s += term_s
+        term_c = -term_c * r * r / ((i + 1) * (i + 2))
+        term_s = -term_s * r * r / ((i + 2) * (i + 3))
+    cs = float([c, -s, -c, s][k % 4])
+    return math.sqrt(-2 * lg) * cs
+
+
+def _check_mismatches_correctly_rounded(got, ref, eq, N, L, seed):
+    import math
+    from oracle import oracle as O
+    factor = math.sqrt(L ** 3 / 2)
+    H = N // 2 + 1
+    bad = np.argwhere(~eq.reshape(got.shape[0], got.shape[1], got.shape[2], 2))
+    for (x, y, z, part) in bad[:200]:
+        m = (x * N + y) * H + z
+        off = 4 * (m - 1) + 2 * part  # DC takes no draws
+        A, B = O.draws(seed, off, 2)
+        want = _cr_normal(A, B) * factor
+        val = got[x, y, z].real if part == 0 else got[x, y, z].imag
+        assert val == want, (x, y, z, part, val, want)
 
 
 def _state_after(oracle, seed, ndraws):
