@@ -52,6 +52,45 @@ def cpu_baseline(min_seconds: float = 10.0) -> dict:
                       "1 thread"}
 
 
+def bench_ensemble(ic, torch, reps: int = 5) -> dict:
+    """96 realizations (32 each at 64^3/250, 128^3/500, 256^3/1000 Mpc/h, seeds
+    1..96, n_s over [0.94, 0.99], FP32, 32 bins) as one graph launch, against
+    the same realizations issued one by one without a graph."""
+    cfgs = ic.ensemble_configs()
+    log("[bench] ensemble: create + capture")
+    ens = ic.Ensemble(cfgs)
+    ens.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ens.enqueue()
+    ens.run()
+    el = (time.perf_counter() - t0) / (reps + 1)
+    ens.close()
+    npart = sum(c.N ** 3 for c in cfgs)
+    balg = sum(b_alg(c.N, 4) for c in cfgs)
+    # unbatched: one context per size, realizations issued in sequence
+    ctxs = {c.N: ic.IC(c) for c in cfgs}
+    stream = torch.cuda.Stream()
+    for c in cfgs:
+        ctxs[c.N].enqueue(c, stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in cfgs:
+        ctxs[c.N].enqueue(c, stream.cuda_stream)
+    torch.cuda.synchronize()
+    el1 = time.perf_counter() - t0
+    for x in ctxs.values():
+        x.close()
+    log(f"[bench] ensemble: {1e3 * el:.1f} ms per graph launch, unbatched {1e3 * el1:.1f} ms")
+    return {"realizations": len(cfgs), "graph_launch_ms": 1e3 * el,
+            "realizations_per_s": len(cfgs) / el, "particles_per_s": npart / el,
+            "unbatched_ms": 1e3 * el1, "speedup_vs_unbatched": el1 / el,
+            "hbm_frac_of_B_alg": balg / el / 1e9 / PEAK_HBM_GBS,
+            "workload": "32 x 64^3/250 + 32 x 128^3/500 + 32 x 256^3/1000 Mpc/h, seeds 1..96, "
+                        "n_s 0.94..0.99, FP32, 2LPT, CIC, P(k) 32 bins"}
+
+
 def pmc_pass(ic, torch, N, L, bins):
     """Dispatches profiled by the counter runs: realization 2 is the measured one."""
     cfg = ic.ICConfig(N=N, boxlen=L, seed=42, bins=bins, precision=ic.MITOS_FP32)
@@ -79,6 +118,8 @@ def main():
     ap.add_argument("--bins", type=int, default=128)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparisons")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 96-realization ensemble (one graph launch)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="one warm-up and one measured FP32 realization, then a plain copy "
                          "of 1 GiB (counter calibration); for rocprofv3 --pmc runs")
@@ -153,6 +194,9 @@ def main():
         psi32 = ctx.displacements()
     ctx.close()
     del ctx
+
+    if not a.no_ensemble:
+        out["ensemble"] = bench_ensemble(ic, torch)
 
     if not a.no_fp64:
         # ---- FP64 fused and plain on the same realization ----------------------------------

@@ -318,6 +318,24 @@ int mitos_ic_profile_entry(mitos_ic_context *ctx, int i, const char **name, floa
 int mitos_ic_run_reference_order(const struct mitos_ic_config *cfg,
                                  struct mitos_ic_result *res, float *X, float *Y, float *Z);
 
+/* ======================================================================== */
+/* Part 4: ensembles -- n realizations of uneven size (any mix of N, boxlen, */
+/* seed, n_s, bins, precision) captured once as one HIP graph and replayed.  */
+/* Realizations of one (N, precision) reuse one set of grids in sequence;    */
+/* groups of different sizes run concurrently.  Each realization's P(k) is   */
+/* identical to mitos_ic_run with the same config.                           */
+/* ======================================================================== */
+typedef struct mitos_ensemble mitos_ensemble;
+
+/* allocates the groups' grids, runs every realization once, captures the graph */
+int mitos_ensemble_create(const struct mitos_ic_config *cfgs, int n, mitos_ensemble **ens);
+/* one graph launch of all realizations; _run waits for it */
+int mitos_ensemble_run(mitos_ensemble *ens);
+int mitos_ensemble_enqueue(mitos_ensemble *ens, void *stream);
+/* P(k) of realization i of the last launch */
+int mitos_ensemble_fetch_pk(mitos_ensemble *ens, int i, struct mitos_ic_result *res);
+int mitos_ensemble_destroy(mitos_ensemble *ens);
+
 #pragma GCC visibility pop
 #ifdef __cplusplus
 }

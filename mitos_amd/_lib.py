@@ -162,6 +162,12 @@ SIGNATURES = [
                                               ctypes.POINTER(ctypes.c_double)]),
     ("mitos_ic_run_reference_order", ctypes.c_int, [ctypes.POINTER(IcConfig),
                                                     ctypes.POINTER(IcResult), vp, vp, vp]),
+    ("mitos_ensemble_create", ctypes.c_int, [ctypes.POINTER(IcConfig), ctypes.c_int,
+                                             ctypes.POINTER(vp)]),
+    ("mitos_ensemble_run", ctypes.c_int, [vp]),
+    ("mitos_ensemble_enqueue", ctypes.c_int, [vp, vp]),
+    ("mitos_ensemble_fetch_pk", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(IcResult)]),
+    ("mitos_ensemble_destroy", ctypes.c_int, [vp]),
 ]
 
 _lib = None

@@ -27,7 +27,5 @@ enum {
     KID_BBKS = 8,
 };
 
-// sigma8 normalisation (host)
-double sigma8_amplitude(double n_s, double gamma, double sigma8, double R);
 
 }  // namespace mitos

@@ -80,10 +80,6 @@ __global__ void k_apply_kernel(cx<TW>* w, const cx<TR>* r, int N, long hs, long 
     w[id] = mk<TW>((TW)(re * kv.re - im * kv.im), (TW)(re * kv.im + im * kv.re));
 }
 
-// Host: device table w[i] = 1/sinc(0.5 k_i boxlen/N)^order for i in [0, N),
-// k_i = freq_index(i)*2pi/boxlen.  The window of include/fft_kernels.h:198-219
-// is separable, 1/W(k) = w[x] w[y] w[z] (k != 0).  Cached per (N, boxlen, order).
-const double* window_table(int N, double boxlen, int order);
 
 // elementwise complex scaling (normalisations)
 template <typename T>

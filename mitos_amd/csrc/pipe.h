@@ -1,0 +1,533 @@
+// The fused IC pipeline (struct Pipe) and its host-side tables, shared by the
+// single-realization API (pipeline.hip) and the ensemble API (ensemble.hip).
+#pragma once
+
+#include <math.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/mitos_hip.h"
+#include "pipeline_kernels.h"
+#include "powerspec.h"
+
+namespace mitos {
+
+
+// ---- spectrum ------------------------------------------------------------------------
+inline double tophat_w(double x) {
+    if (x < 1e-3) return 1.0 - x * x / 10.0;
+    return 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
+}
+
+// sigma_R^2 = int dk/(2 pi^2) k^2 P(k) W(kR)^2 with P = A k^n_s T_BBKS^2, Simpson in ln k
+inline double sigma8_amplitude(double n_s, double gamma, double sigma8, double R) {
+    const double lo = log(1e-6), hi = log(1e3);
+    const int n = 200000;
+    const double h = (hi - lo) / n;
+    double sum = 0.0;
+    for (int i = 0; i <= n; i++) {
+        const double k = exp(lo + i * h);
+        const double T = bbks_T(k, gamma);
+        const double W = tophat_w(k * R);
+        const double f = k * k * k * pow(k, n_s) * T * T * W * W / (2.0 * M_PI * M_PI);
+        const double w = (i == 0 || i == n) ? 1.0 : ((i & 1) ? 4.0 : 2.0);
+        sum += w * f;
+    }
+    return sigma8 * sigma8 / (sum * h / 3.0);
+}
+
+// Host: device table w[i] = 1/sinc(0.5 k_i boxlen/N)^order for i in [0, N),
+// k_i = freq_index(i)*2pi/boxlen.  The window of include/fft_kernels.h:198-219
+// is separable, 1/W(k) = w[x] w[y] w[z] (k != 0).  Cached per (N, boxlen, order).
+inline const double* window_table(int N, double boxlen, int order) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, double, int>, double*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto key = std::make_tuple(dev, N, boxlen, order);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> h(N);
+    const double dk = 2 * M_PI / boxlen;
+    for (int i = 0; i < N; i++) {
+        const double k = freq_index(i, N) * dk;
+        const double w = sinc_d(0.5 * k * boxlen / N);
+        h[i] = 1.0 / pow(w, order);
+    }
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * N) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
+// Device table S[n] = 3/4 + cos(2 pi n/N)/4, n in [0, N): the DFT of the
+// periodic lattice stencil (1/8, 3/4, 1/8).  Cached per (device, N, type).
+template <typename V>
+inline const V* smooth_table(int N) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, V*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto key = std::make_pair(dev, N);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<V> h(N);
+    for (int n = 0; n < N; n++) {
+        // exact values at the quarter points
+        double c;
+        if (4 * n == N || 4 * n == 3 * N) c = 0.0;
+        else if (2 * n == N) c = -1.0;
+        else c = cos(2.0 * M_PI * n / N);
+        h[n] = (V)(0.75 + 0.25 * c);
+    }
+    V* d = nullptr;
+    if (hipMalloc(&d, sizeof(V) * N) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(V) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
+inline double amplitude_cached(double n_s, double gamma, double sigma8) {
+    static std::mutex mu;
+    static std::map<std::tuple<double, double, double>, double> cache;
+    auto key = std::make_tuple(n_s, gamma, sigma8);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const double A = sigma8_amplitude(n_s, gamma, sigma8, 8.0);
+    cache[key] = A;
+    return A;
+}
+
+// Device table over the integer n^2 = |k|^2/dk^2 of the amplitude taking white
+// noise to phi1(k) = -delta(k)/k^2, delta = noise * sqrt(A k^n_s) T_BBKS(k)
+// (the primordial kernel of src/mitos.c:224, the transfer kernel of
+// src/generate_grids.c:52 and kernel_inv_poisson of src/2lpt.c:75).
+inline std::vector<double> phi1_host(int N, double boxlen, double A, double n_s, double gamma) {
+    const long n2max = 3L * (N / 2) * (N / 2);
+    std::vector<double> h(n2max + 1);
+    const double dk = 2 * M_PI / boxlen;
+    h[0] = 0.0;
+    for (long n2 = 1; n2 <= n2max; n2++) {
+        const double k = sqrt((double)n2) * dk;
+        const double p = sqrt(A * pow(k, n_s));
+        const double t = bbks_T(k, gamma);
+        h[n2] = (p * t) * (-1.0 / k / k);
+    }
+    return h;
+}
+
+inline const double* phi1_table(int N, double boxlen, double A, double n_s, double gamma) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, double, double, double, double>, double*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto key = std::make_tuple(dev, N, boxlen, A, n_s, gamma);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (cache.size() > 64) {  // bounded: one entry per (size, spectrum)
+        hipDeviceSynchronize();
+        for (auto& kv : cache) hipFree(kv.second);
+        cache.clear();
+    }
+    const std::vector<double> h = phi1_host(N, boxlen, A, n_s, gamma);
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * h.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
+template <typename T>
+__global__ void k_fill_u64(unsigned long long* p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ---- launch helpers for the fused kernels ---------------------------------------------
+// Line configuration and tile width of the register-heavy multi-transform
+// kernels: 8 (or 12) values per thread and 64-byte tile rows, so that the
+// input tile, the transformed copy and the radix temporaries stay in VGPRs.
+template <typename T, int L>
+struct HeavyCfg {
+    typedef LineCfgS<L> CFG;
+    static constexpr int E = CFG::E;
+    static constexpr int TPL = L / E;
+    static constexpr int C0 = 64 / (2 * (int)sizeof(T));
+    static constexpr int C = (TPL * C0 <= 1024) ? C0 : 1024 / TPL;
+};
+template <typename T, int L, int AXIS, int DIR, int NIN, typename CODES,
+          typename RAD = typename HeavyCfg<T, L>::CFG::R>
+struct MultiK;
+template <typename T, int L, int AXIS, int DIR, int NIN, typename CODES, int... Rs>
+struct MultiK<T, L, AXIS, DIR, NIN, CODES, Radices<Rs...>> {
+    static int go(const MultiPtrs<T, NIN, CODES::n>& P, long hs, double dk, hipStream_t s) {
+        constexpr int C = HeavyCfg<T, L>::C;
+        constexpr int E = HeavyCfg<T, L>::E;
+        constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
+        const int ncols = L / 2 + 1;
+        const cx<T>* tw = twiddles<T>(L);
+        if (!tw) return MITOS_ERR_ALLOC;
+        hipLaunchKernelGGL((k_multi_strided<T, L, E, C, AXIS, DIR, SPLIT, NIN, CODES, Rs...>),
+                           dim3((ncols + C - 1) / C, L), dim3(L / E * C), 0, s, P, hs, ncols, dk,
+                           tw);
+        return launch_status();
+    }
+};
+
+template <typename T, int N, typename RAD = typename LineCfgS<N / 2>::R>
+struct HessZK;
+template <typename T, int N, int... Rs>
+struct HessZK<T, N, Radices<Rs...>> {
+    static int go(const MultiPtrs<T, 6, 1>& P, long hs, double dk, T scale, hipStream_t s) {
+        constexpr int L = N / 2;
+        constexpr int E = LineCfgS<L>::E;
+        constexpr int TPL = L / E;
+        constexpr int RW = RowsPerBlock<TPL>::value;
+        const long nrows = (long)N * N;
+        const cx<T>* tw = twiddles<T>(L);
+        const cx<T>* twN = twiddles<T>(N);
+        if (!tw || !twN) return MITOS_ERR_ALLOC;
+        hipLaunchKernelGGL((k_hess_z<T, L, E, RW, Rs...>), dim3(nblk(nrows, RW)), dim3(TPL * RW),
+                           0, s, P, nrows, hs, dk, scale, tw, twN);
+        return launch_status();
+    }
+};
+
+template <typename T, int N, typename RAD = typename HeavyCfg<T, N>::CFG::R>
+struct CombineK;
+template <typename T, int N, int... Rs>
+struct CombineK<T, N, Radices<Rs...>> {
+    static int go(const cx<T>* D, const cx<T>* K, cx<T>* C0, cx<T>* C1, long hs,
+                  const CombineParams& cp, hipStream_t s) {
+        constexpr int L = N;
+        constexpr int C = HeavyCfg<T, L>::C;
+        constexpr int E = HeavyCfg<T, L>::E;
+        constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
+        const int ncols = L / 2 + 1;
+        const cx<T>* tw = twiddles<T>(L);
+        if (!tw) return MITOS_ERR_ALLOC;
+        hipLaunchKernelGGL((k_x_combine<T, L, E, C, SPLIT, Rs...>),
+                           dim3((ncols + C - 1) / C, L), dim3(L / E * C), 0, s, D, K, C0, C1, hs,
+                           ncols, cp, tw);
+        return launch_status();
+    }
+};
+
+// ---- the fused pipeline ------------------------------------------------------------------
+template <typename T>
+struct Pipe {
+    int N = 0;
+    long hs = 0;      // complex row stride (elements): N/2+1 rounded up to 128 bytes
+    long csize = 0;   // elements of one complex grid
+    cx<T>* K = nullptr;
+    cx<T>* A = nullptr;  // 3 complex grids
+    cx<T>* B = nullptr;  // 6 complex grids
+    float *X = nullptr, *Y = nullptr, *Z = nullptr;
+    double* pk_sums = nullptr;  // device [bins][2]
+    long long* pk_obs = nullptr;
+    double* pk_part = nullptr;  // per-work-group P(k) partials of this pipeline
+    long long* pk_cnt = nullptr;
+    int bins_alloc = 0;
+    hipEvent_t ev[9] = {};
+    double amplitude = 0;
+    // optional per-kernel profile of the last run: name, algorithmic bytes,
+    // start/stop events on the run's stream
+    bool prof = false;
+    int nprof = 0;
+    const char* pname[32];
+    double pbytes[32];
+    hipEvent_t pev[33];
+    template <typename F>
+    int timed(hipStream_t s, const char* name, double bytes, F&& f) {
+        if (prof && nprof < 32) {
+            hipEventRecord(pev[nprof], s);
+            const int rc = f();
+            pname[nprof] = name;
+            pbytes[nprof] = bytes;
+            nprof++;
+            hipEventRecord(pev[nprof], s);
+            return rc;
+        }
+        return f();
+    }
+
+    cx<T>* a(int i) { return A + (size_t)i * csize; }
+    cx<T>* b(int i) { return B + (size_t)i * csize; }
+    T* psi(int i) { return reinterpret_cast<T*>(b(3 + i)); }
+    unsigned long long* cic() { return reinterpret_cast<unsigned long long*>(b(0)); }
+    // fixed-point unit of the lattice deposit (float weights in the FP32 pipeline)
+    static int cic_shift() { return sizeof(T) == 4 ? MITOS_CIC_SHIFT_F32 : MITOS_CIC_SHIFT; }
+
+    int create(int n) {
+        N = n;
+        const long H = N / 2 + 1;
+        const long q = 128 / sizeof(cx<T>);
+        hs = (H + q - 1) / q * q;
+        csize = (long)N * N * hs;
+        // a displacement grid (rows of N + MITOS_PSI_PAD reals) fits in a complex grid
+        if (N + MITOS_PSI_PAD > 2 * hs) return MITOS_ERR_UNSUPPORTED;
+        const long long np = (long long)N * N * N;
+        MITOS_HIP_CHECK(hipMalloc(&K, sizeof(cx<T>) * csize));
+        MITOS_HIP_CHECK(hipMalloc(&A, sizeof(cx<T>) * csize * 3));
+        MITOS_HIP_CHECK(hipMalloc(&B, sizeof(cx<T>) * csize * 6));
+        MITOS_HIP_CHECK(hipMalloc(&X, sizeof(float) * np));
+        MITOS_HIP_CHECK(hipMalloc(&Y, sizeof(float) * np));
+        MITOS_HIP_CHECK(hipMalloc(&Z, sizeof(float) * np));
+        for (int i = 0; i < 9; i++) MITOS_HIP_CHECK(hipEventCreate(&ev[i]));
+        for (int i = 0; i < 33; i++) MITOS_HIP_CHECK(hipEventCreate(&pev[i]));
+        return MITOS_OK;
+    }
+
+    void destroy() {
+        hipDeviceSynchronize();
+        hipFree(K);
+        hipFree(A);
+        hipFree(B);
+        hipFree(X);
+        hipFree(Y);
+        hipFree(Z);
+        hipFree(pk_sums);
+        hipFree(pk_obs);
+        hipFree(pk_part);
+        hipFree(pk_cnt);
+        for (int i = 0; i < 9; i++)
+            if (ev[i]) hipEventDestroy(ev[i]);
+        for (int i = 0; i < 33; i++)
+            if (pev[i]) hipEventDestroy(pev[i]);
+    }
+
+    int ensure_bins(int bins) {
+        if (bins <= bins_alloc) return MITOS_OK;
+        hipFree(pk_sums);
+        hipFree(pk_obs);
+        hipFree(pk_part);
+        hipFree(pk_cnt);
+        const int nb = pk_blocks((long)N * N * (N / 2 + 1));
+        MITOS_HIP_CHECK(hipMalloc(&pk_sums, sizeof(double) * 2 * bins));
+        MITOS_HIP_CHECK(hipMalloc(&pk_obs, sizeof(long long) * bins));
+        MITOS_HIP_CHECK(hipMalloc(&pk_part, sizeof(double) * 4 * bins * nb));
+        MITOS_HIP_CHECK(hipMalloc(&pk_cnt, sizeof(long long) * bins * nb));
+        bins_alloc = bins;
+        return MITOS_OK;
+    }
+
+    // One realization on stream s.  amp_tab: device phi1 amplitude table to use
+    // instead of the cached one; pk_dst/obs_dst: device destinations of the
+    // P(k) sums and counts instead of pk_sums/pk_obs.  No allocation and no
+    // host synchronisation happen here once the tables of cfg exist, so the
+    // call can be captured into a graph.
+    int enqueue(const mitos_ic_config& cfg, hipStream_t s, bool timing,
+                const double* amp_tab = nullptr, double* pk_dst = nullptr,
+                long long* obs_dst = nullptr) {
+        const double L = cfg.boxlen;
+        const double dk = 2 * M_PI / L;
+        const double boxvol = L * L * L;
+        const double sfwd = boxvol / ((long long)N * N * N);
+        const long long np = (long long)N * N * N;
+        // algorithmic bytes: one half-complex grid, one real grid
+        const double G = (double)N * N * (N / 2 + 1) * sizeof(cx<T>);
+        const double Rg = (double)np * sizeof(T);
+        MITOS_TRY(ensure_bins(cfg.bins));
+        amplitude = amplitude_cached(cfg.n_s, cfg.gamma, cfg.sigma8);
+        nprof = 0;
+        auto mark = [&](int i) {
+            if (timing) hipEventRecord(ev[i], s);
+        };
+        mark(0);
+        // 1. noise -> phi1(k), Hermitian planes (src/grf.c, src/2lpt.c:75)
+        Xo s0 = xo_seed(cfg.seed);
+        NoiseJumps J;
+        MITOS_TRY(noise_jumps(N, s0, &J));
+        const double* amp = amp_tab ? amp_tab : phi1_table(N, L, amplitude, cfg.n_s, cfg.gamma);
+        if (!amp) return MITOS_ERR_ALLOC;
+        // FP64 keeps correctly rounded Box-Muller; FP32 rounds every value to float
+        MITOS_TRY(timed(s, "grf_noise", G, [&] {
+            return launch_grf<T, AmpTable, sizeof(T) == 8>(K, hs, N, L, 1, 0, J, AmpTable{amp}, s);
+        }));
+        MITOS_TRY(timed(s, "hermitian_planes", 0.0,
+                        [&] { return launch_hermitian<T>(K, hs, N, 1, 0, s); }));
+        mark(1);
+        return with_size(N, [&]<int NN>() -> int {
+            cx<T>* D = a(0);
+            if (cfg.run2lpt) {
+                // 2. x pass: A0 = X(phi1), A1 = X(i kx phi1), A2 = X(-kx^2 phi1)
+                MultiPtrs<T, 1, 3> p2;
+                p2.in[0] = K;
+                p2.out[0] = a(0);
+                p2.out[1] = a(1);
+                p2.out[2] = a(2);
+                MITOS_TRY(timed(s, "hessian_x_1to3", 4 * G, [&] {
+                    return MultiK<T, NN, 0, 1, 1,
+                                  OutCodes<MITOS_OUT(0, 0), MITOS_OUT(0, 1), MITOS_OUT(0, 2)>>::go(
+                        p2, hs, dk, s);
+                }));
+                // 3. y pass: six Hessian factors
+                MultiPtrs<T, 3, 6> p3;
+                p3.in[0] = a(0);
+                p3.in[1] = a(1);
+                p3.in[2] = a(2);
+                for (int i = 0; i < 6; i++) p3.out[i] = b(i);
+                MITOS_TRY(timed(s, "hessian_y_3to6", 9 * G, [&] {
+                    return MultiK<T, NN, 1, 1, 3,
+                                  OutCodes<MITOS_OUT(0, 0), MITOS_OUT(0, 1), MITOS_OUT(0, 2),
+                                           MITOS_OUT(1, 0), MITOS_OUT(1, 1), MITOS_OUT(2, 0)>>::go(
+                        p3, hs, dk, s);
+                }));
+                mark(2);
+                // 4. z pass: delta2 and its r2c row (src/2lpt.c:84-116, poisson.c:61)
+                MultiPtrs<T, 6, 1> p4;
+                for (int i = 0; i < 6; i++) p4.in[i] = b(i);
+                p4.out[0] = D;
+                MITOS_TRY(timed(s, "hessian_z_delta2", 7 * G, [&] {
+                    return HessZK<T, NN>::go(p4, hs, dk, (T)(1.0 / boxvol), s);
+                }));
+                // 5. y pass forward of delta2
+                MITOS_TRY(timed(s, "delta2_y_fwd", 2 * G, [&] {
+                    return strided_pass<T, NN, 1, -1>(D, D, hs, 1, 0, OpNone(), s);
+                }));
+            } else {
+                D = K;  // Zel'dovich: the second-order term is multiplied by zero
+                mark(2);
+            }
+            mark(3);
+            // 6. x pass: Poisson + combine + TSC window + first inverse pass
+            CombineParams cp;
+            cp.dk = dk;
+            cp.sD = sfwd;
+            cp.f1 = cfg.f1;
+            cp.f2 = cfg.run2lpt ? cfg.f2 : 0.0;
+            cp.win = window_table(NN, L, 3);
+            if (!cp.win) return MITOS_ERR_ALLOC;
+            // FP32 applies the lattice TSC stencils in k-space (k_displace_lattice_s)
+            constexpr bool STENCIL = sizeof(T) == 4;
+            cp.smooth = nullptr;
+            if (STENCIL) {
+                cp.smooth = smooth_table<double>(NN);
+                if (!cp.smooth) return MITOS_ERR_ALLOC;
+            }
+            MITOS_TRY(timed(s, "combine_x", 4 * G, [&] {
+                return CombineK<T, NN>::go(D, K, a(1), a(2), hs, cp, s);
+            }));
+            // 7. y pass: E1 = Y(i ky C0) (FP32: i ky S(ky) C0), E2 = Y(C0), E0 = Y(C1)
+            MultiPtrs<T, 2, 3> p7;
+            p7.in[0] = a(1);
+            p7.in[1] = a(2);
+            p7.out[0] = b(1);
+            p7.out[1] = b(2);
+            p7.out[2] = b(0);
+            if (STENCIL) {
+                p7.smooth = smooth_table<T>(NN);
+                if (!p7.smooth) return MITOS_ERR_ALLOC;
+            }
+            constexpr int E1CODE = STENCIL ? MITOS_OUT(0, 3) : MITOS_OUT(0, 1);
+            MITOS_TRY(timed(s, "displacement_y_2to3", 5 * G, [&] {
+                return MultiK<T, NN, 1, 1, 2,
+                              OutCodes<E1CODE, MITOS_OUT(0, 0), MITOS_OUT(1, 0)>>::go(p7, hs, dk,
+                                                                                     s);
+            }));
+            // 8. z c2r: displacement grids (src/mitos.c:404-416)
+            const T cs = (T)(1.0 / boxvol);
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
+                return zc2r_pass<T, NN, decltype(OpNone()), MITOS_PSI_HO>(
+                    b(0), psi(0), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpNone(), s);
+            }));
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
+                return zc2r_pass<T, NN, decltype(OpNone()), MITOS_PSI_HO>(
+                    b(1), psi(1), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpNone(), s);
+            }));
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
+                return zc2r_pass<T, NN, decltype(OpIkz{dk}), MITOS_PSI_HO>(
+                    b(2), psi(2), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpIkz{dk}, s);
+            }));
+            mark(4);
+            // 9-10. lattice + sequential TSC displacement + wrap + CIC deposit
+            const float spacing = (float)L / NN;
+            MITOS_TRY(timed(s, "cic_clear", 8.0 * np, [&] {
+                hipLaunchKernelGGL((k_fill_u64<T>), dim3(nblk(np)), dim3(256), 0, s, cic(),
+                                   (long)np);
+                return launch_status();
+            }));
+            mark(5);
+            // FP32 moves the particles in float arithmetic, FP64 in double
+            typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
+            {
+                // reads 3 displacement grids, writes 3 float coordinates
+                MITOS_TRY(timed(s, "displace_tsc", 3 * Rg + 12.0 * np, [&] {
+                    constexpr int NT = NN >= 256 ? 256 : 64;
+                    const int nch = (NN + NT - 1) / NT;
+                    const dim3 grid((unsigned)((long)NN * NN * nch));
+                    const unsigned rs = (unsigned)(NN + MITOS_PSI_PAD);
+                    if constexpr (STENCIL) {
+                        constexpr int TX = NN % 16 == 0 ? 16 : 8;
+                        hipLaunchKernelGGL((k_displace_lattice_s<NT, TX>), grid, dim3(NT), 0, s,
+                                           (const float*)psi(0), (const float*)psi(1),
+                                           (const float*)psi(2), rs, NN, nch, (float)L, spacing,
+                                           L, X, Y, Z);
+                    } else {
+                        hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,
+                                           psi(0), psi(1), psi(2), rs, NN, nch, L, spacing,
+                                           (float)(NN / L), X, Y, Z);
+                    }
+                    return launch_status();
+                }));
+                // reads 3 float coordinates, writes the grid
+                MITOS_TRY(timed(s, "cic_deposit", 12.0 * np + 8.0 * np, [&] {
+                    if (NN % 32 == 0) {
+                        hipLaunchKernelGGL((k_cic_lattice<M, MITOS_CIC_TILE>),
+                                           dim3((NN / 8) * (NN / 16) * (NN / 32)), dim3(512), 0,
+                                           s, X, Y, Z, NN, L, cic());
+                        return launch_status();
+                    }
+                    hipLaunchKernelGGL(k_cic<M>, dim3(nblk(np)), dim3(256), 0, s, X, Y, Z, np, NN, L,
+                                       cic());
+                    return launch_status();
+                }));
+            }
+            mark(6);
+            // 11. density r2c (overdensity on load), 12. P(k) with CIC window
+            CicLoader<T> ld{cic(), (long)NN, ldexp(1.0, -cic_shift())};
+            MITOS_TRY(timed(s, "density_z_r2c", 8.0 * np + G, [&] {
+                return zr2c_pass<T, NN>(ld, K, hs, 1, 0, OpNone(), s);
+            }));
+            MITOS_TRY(timed(s, "density_y_fwd", 2 * G, [&] {
+                return strided_pass<T, NN, 1, -1>(K, K, hs, 1, 0, OpNone(), s);
+            }));
+            MITOS_TRY(timed(s, "density_x_fwd", 2 * G, [&] {
+                return strided_pass<T, NN, 0, -1>(K, K, hs, 1, 0, OpNone(), s);
+            }));
+            const double* w2 = window_table(NN, L, 2);
+            if (!w2) return MITOS_ERR_ALLOC;
+            PkCicTransform xf{sfwd, w2, NN};
+            MITOS_TRY(timed(s, "pk_bins", G, [&] {
+                return pk_run<T, PkCicTransform>(K, K, NN, hs, L, cfg.bins, xf, nullptr, nullptr,
+                                                 s, pk_dst ? pk_dst : pk_sums,
+                                                 obs_dst ? obs_dst : pk_obs, pk_part, pk_cnt);
+            }));
+            mark(7);
+            return MITOS_OK;
+        });
+    }
+};
+
+}  // namespace mitos

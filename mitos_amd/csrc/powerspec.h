@@ -174,9 +174,12 @@ inline int pk_blocks(long nmodes) {
 
 // Binned P(k) of two half-complex grids with row stride hs; host outputs
 // sums (P, k) and counts.  `xf` transforms each value before the product.
+// part_ws/cnt_ws: caller-owned partials (pk_blocks x bins entries), else the
+// library workspace.
 template <typename T, typename XF>
 inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen, int bins, XF xf,
-           double* sums, long long* obs, hipStream_t s, double* dev_sums, long long* dev_obs) {
+                  double* sums, long long* obs, hipStream_t s, double* dev_sums,
+                  long long* dev_obs, double* part_ws = nullptr, long long* cnt_ws = nullptr) {
     int err = MITOS_OK;
     const int* edges = pk_bin_edges(N, boxlen, bins, &err);
     if (!edges) return err;
@@ -184,11 +187,14 @@ inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen,
     const int nblk = pk_blocks(nmodes);
     const size_t pbytes = sizeof(double) * 4 * bins * nblk;
     const size_t cbytes = sizeof(long long) * bins * nblk;
-    char* ws = (char*)workspace(2, pbytes + cbytes + sizeof(double) * 2 * bins +
-                                       sizeof(long long) * bins);
-    if (!ws) return MITOS_ERR_ALLOC;
-    double* part = (double*)ws;
-    long long* cnt = (long long*)(ws + pbytes);
+    char* ws = nullptr;
+    if (!part_ws || !dev_sums) {
+        ws = (char*)workspace(2, pbytes + cbytes + sizeof(double) * 2 * bins +
+                                     sizeof(long long) * bins);
+        if (!ws) return MITOS_ERR_ALLOC;
+    }
+    double* part = part_ws ? part_ws : (double*)ws;
+    long long* cnt = part_ws ? cnt_ws : (long long*)(ws + pbytes);
     double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes + cbytes);
     long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + cbytes + sizeof(double) * 2 * bins);
     const size_t shm =

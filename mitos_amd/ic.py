@@ -303,3 +303,57 @@ def run_reference_order(cfg: ICConfig, positions: bool = False) -> dict:
     if positions:
         out.update(X=X, Y=Y, Z=Z)
     return out
+
+
+class Ensemble:
+    """n realizations of uneven size as one captured HIP graph
+    (mitos_ensemble_create / run / fetch_pk / destroy)."""
+
+    def __init__(self, cfgs: list):
+        self.cfgs = list(cfgs)
+        arr = (_lib.IcConfig * len(self.cfgs))(*[c.c() for c in self.cfgs])
+        self.h = ctypes.c_void_p()
+        check(lib().mitos_ensemble_create(arr, len(self.cfgs), ctypes.byref(self.h)),
+              "mitos_ensemble_create")
+
+    def run(self) -> None:
+        check(lib().mitos_ensemble_run(self.h), "mitos_ensemble_run")
+
+    def enqueue(self, stream: int | None = None) -> None:
+        check(lib().mitos_ensemble_enqueue(self.h, stream), "mitos_ensemble_enqueue")
+
+    def pk(self, i: int) -> dict:
+        bins = self.cfgs[i].bins
+        k = np.empty(bins)
+        P = np.empty(bins)
+        obs = np.empty(bins, np.int64)
+        res = _lib.IcResult(k.ctypes.data_as(_lib.c_double_p), P.ctypes.data_as(_lib.c_double_p),
+                            obs.ctypes.data_as(_lib.c_ll_p), 0.0)
+        check(lib().mitos_ensemble_fetch_pk(self.h, i, ctypes.byref(res)),
+              "mitos_ensemble_fetch_pk")
+        return {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude}
+
+    def close(self) -> None:
+        if self.h:
+            lib().mitos_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ensemble_configs(scale: int = 1) -> list:
+    """The 96-realization ensemble: 32 each at 64^3/250, 128^3/500 and
+    256^3/1000 Mpc/h, seeds 1..96, n_s uniform over [0.94, 0.99] in seed
+    order, FP32, 32 bins.  `scale` divides every N (smaller test ensembles)."""
+    out = []
+    for i in range(96):
+        g = i // 32
+        N = (64, 128, 256)[g] // scale
+        L = (250.0, 500.0, 1000.0)[g]
+        out.append(ICConfig(N=N, boxlen=L, seed=i + 1, n_s=0.94 + 0.05 * i / 95, bins=32,
+                            run2lpt=1, precision=MITOS_FP32))
+    return out
