@@ -164,6 +164,8 @@ def main():
     ms = 1e3 * el / a.steps
     out["value"] = N ** 3 * a.steps / el
     out["ms_per_step"] = ms
+    out["fft_schedules"] = {k: {"variant": v, "ms_per_variant": ms}
+                            for k, (v, ms) in ctx.schedules().items()}
     # one more run with the per-kernel profile and the P(k) result
     ctx.profile(True)
     r32 = ctx.run(cfg)

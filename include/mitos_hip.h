@@ -307,6 +307,16 @@ int mitos_ic_displacements(mitos_ic_context *ctx, void *psi_device);
 int mitos_ic_ids(mitos_ic_context *ctx, long long *ids_device);
 int mitos_ic_destroy(mitos_ic_context *ctx);
 
+/* FFT schedule picked by the autotuner at mitos_ic_create for pass `kind`
+ * (0: y forward, 1: x forward, 2: z c2r, 3: z r2c): returns the variant
+ * (0..2) and fills ms[3] with the timed milliseconds of each variant (all 0
+ * when MITOS_FFT_SCHEDULE=default disabled the timing); -1 on bad arguments */
+int mitos_ic_schedule(mitos_ic_context *ctx, int kind, float *ms);
+/* pin the schedule variants[4] (per kind as above) for contexts and ensembles
+ * of size N and this precision created afterwards on the current device,
+ * instead of timing them */
+int mitos_set_fft_schedule(int N, int precision, const int *variants);
+
 /* per-kernel profile of the following runs (HIP events on the run's stream):
  * mitos_ic_profile_entry returns the number of entries and fills entry i
  * (kernel name, milliseconds, algorithmic bytes) when 0 <= i < count */

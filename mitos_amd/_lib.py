@@ -156,6 +156,9 @@ SIGNATURES = [
     ("mitos_ic_displacements", ctypes.c_int, [vp, vp]),
     ("mitos_ic_ids", ctypes.c_int, [vp, vp]),
     ("mitos_ic_destroy", ctypes.c_int, [vp]),
+    ("mitos_ic_schedule", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    ("mitos_set_fft_schedule", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
     ("mitos_ic_profile", ctypes.c_int, [vp, ctypes.c_int]),
     ("mitos_ic_profile_entry", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                               ctypes.POINTER(ctypes.c_float),

@@ -6,8 +6,9 @@
 // different sizes overlap on the GPU while realizations of one group follow
 // each other.  Every realization has its own spectrum table and its own
 // P(k) slot, so the captured sequence needs no host work between
-// realizations.  The graph is captured once, after one eager pass has made
-// every cached table (twiddles, windows, stencils, bin edges), and replayed by
+// realizations.  The FFT schedules of each group are autotuned first; the
+// graph is captured once, after one eager pass has made every cached table
+// (twiddles, windows, stencils, bin edges), and replayed by
 // mitos_ensemble_run.
 #include <algorithm>
 #include <memory>
@@ -109,10 +110,12 @@ static int ensemble_build(mitos_ensemble* e) {
             g.pf = new Pipe<float>();
             MITOS_TRY(g.pf->create(g.N));
             MITOS_TRY(g.pf->ensure_bins(e->bmax));
+            MITOS_TRY(g.pf->autotune(g.stream));
         } else {
             g.pd = new Pipe<double>();
             MITOS_TRY(g.pd->create(g.N));
             MITOS_TRY(g.pd->ensure_bins(e->bmax));
+            MITOS_TRY(g.pd->autotune(g.stream));
         }
     }
     e->amplitude.resize(n);

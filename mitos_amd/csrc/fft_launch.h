@@ -4,6 +4,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "fft_passes.h"
 
@@ -171,6 +172,151 @@ int zr2c_pass(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op, 
                                                              out, nrows, N, hs, outb, tw, twN,
                                                              op);
     return launch_status();
+}
+
+// ---- tunable schedules ---------------------------------------------------------
+// Variants of the plain passes that the autotuner (autotune.h) times on the
+// device and picks from, per (device, N, precision):
+//   strided  0: LineCfg radices, 128-byte column tile
+//            1: LineCfgS radices (fewer values per thread), the widest tile
+//               of at most 1024 threads
+//            2: LineCfg radices, 256-byte column tile when it fits
+//   z passes 0: LineCfg radices, RowsPerBlock rows per work-group
+//            1: LineCfgS radices, RowsPerBlock rows of its thread count
+//            2: LineCfg radices, twice the rows when it fits
+constexpr int kFftVariants = 3;
+
+template <typename T, int L, int V>
+struct StridedSched {
+    typedef typename std::conditional<V == 1, LineCfgS<L>, LineCfg<L>>::type CFG;
+    static constexpr int TPL = L / CFG::E;
+    static constexpr int CB = (TPL * ColTile<T>::C <= 1024) ? ColTile<T>::C : 1024 / TPL;
+    // the wider tile only while one exchange buffer stays within 80 KiB of LDS
+    static constexpr int C2 = 2 * ColTile<T>::C;
+    static constexpr bool WIDE = V == 2 && TPL * C2 <= 1024 &&
+                                 (long)L * (C2 + 1) * (long)sizeof(T) <= 81920;
+    static constexpr int C = WIDE ? C2 : CB;
+};
+
+template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, typename CFG,
+          typename RAD = typename CFG::R>
+struct StridedKV;
+template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, typename CFG,
+          int... Rs>
+struct StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, CFG, Radices<Rs...>> {
+    static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
+                   int ncols, long bstride, const cx<T>* tw, OP op) {
+        hipLaunchKernelGGL((k_pass_strided<T, L, CFG::E, C, AXIS, DIR, SPLIT, OP, Rs...>), g, b, 0,
+                           s, in, out, hs, ncols, bstride, tw, op);
+    }
+};
+
+template <typename T, int L, int AXIS, int DIR, int V, typename OP>
+int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride, OP op,
+                   hipStream_t s) {
+    typedef StridedSched<T, L, V> S;
+    constexpr int C = S::C;
+    constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
+    const int ncols = L / 2 + 1;
+    const cx<T>* tw = twiddles<T>(L);
+    if (!tw) return MITOS_ERR_ALLOC;
+    dim3 grid((ncols + C - 1) / C, L, nbatch);
+    StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, typename S::CFG>::go(grid, dim3(S::TPL * C), s, in,
+                                                                  out, hs, ncols, bstride, tw, op);
+    return launch_status();
+}
+
+// strided pass with a run-time schedule variant
+template <typename T, int L, int AXIS, int DIR, typename OP>
+int strided_pass_tuned(int variant, const cx<T>* in, cx<T>* out, long hs, int nbatch,
+                       long bstride, OP op, hipStream_t s) {
+    switch (variant) {
+        case 1: return strided_pass_v<T, L, AXIS, DIR, 1>(in, out, hs, nbatch, bstride, op, s);
+        case 2: return strided_pass_v<T, L, AXIS, DIR, 2>(in, out, hs, nbatch, bstride, op, s);
+        default: return strided_pass_v<T, L, AXIS, DIR, 0>(in, out, hs, nbatch, bstride, op, s);
+    }
+}
+
+template <typename T, int L, int V>
+struct ZSched {
+    typedef typename std::conditional<V == 1, LineCfgS<L>, LineCfg<L>>::type CFG;
+    static constexpr int TPL = L / CFG::E;
+    static constexpr int RW0 = RowsPerBlock<TPL>::value * (V == 2 ? 2 : 1);
+    static constexpr int RW = (TPL * RW0 <= 1024) ? RW0 : RowsPerBlock<TPL>::value;
+};
+
+template <typename T, int L, int RW, int HO, typename OP, typename CFG,
+          typename RAD = typename CFG::R>
+struct ZC2RKV;
+template <typename T, int L, int RW, int HO, typename OP, typename CFG, int... Rs>
+struct ZC2RKV<T, L, RW, HO, OP, CFG, Radices<Rs...>> {
+    template <typename... A>
+    static void go(dim3 g, dim3 b, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_zc2r<T, L, CFG::E, RW, HO, OP, Rs...>), g, b, 0, s, a...);
+    }
+};
+
+template <typename T, int N, int V, int HO, typename OP>
+int zc2r_pass_v(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, long outb,
+                T scale, OP op, hipStream_t s) {
+    if (rs < N + (HO > 0 ? HO + 1 : 0)) return MITOS_ERR_ARG;
+    constexpr int L = N / 2;
+    typedef ZSched<T, L, V> S;
+    const long nrows = (long)N * N;
+    const cx<T>* tw = twiddles<T>(L);
+    const cx<T>* twN = twiddles<T>(N);
+    if (!tw || !twN) return MITOS_ERR_ALLOC;
+    dim3 grid((unsigned)((nrows + S::RW - 1) / S::RW), nbatch);
+    ZC2RKV<T, L, S::RW, HO, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, in, out,
+                                                     nrows, N, hs, rs, inb, outb, tw, twN, scale,
+                                                     op);
+    return launch_status();
+}
+
+template <typename T, int N, int HO, typename OP>
+int zc2r_pass_tuned(int variant, const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb,
+                    long outb, T scale, OP op, hipStream_t s) {
+    switch (variant) {
+        case 1: return zc2r_pass_v<T, N, 1, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
+        case 2: return zc2r_pass_v<T, N, 2, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
+        default: return zc2r_pass_v<T, N, 0, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
+    }
+}
+
+template <typename T, int L, int RW, typename LOADER, typename OP, typename CFG,
+          typename RAD = typename CFG::R>
+struct ZR2CKV;
+template <typename T, int L, int RW, typename LOADER, typename OP, typename CFG, int... Rs>
+struct ZR2CKV<T, L, RW, LOADER, OP, CFG, Radices<Rs...>> {
+    template <typename... A>
+    static void go(dim3 g, dim3 b, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_zr2c<T, L, CFG::E, RW, LOADER, OP, Rs...>), g, b, 0, s, a...);
+    }
+};
+
+template <typename T, int N, int V, typename LOADER, typename OP>
+int zr2c_pass_v(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op,
+                hipStream_t s) {
+    constexpr int L = N / 2;
+    typedef ZSched<T, L, V> S;
+    const long nrows = (long)N * N;
+    const cx<T>* tw = twiddles<T>(L);
+    const cx<T>* twN = twiddles<T>(N);
+    if (!tw || !twN) return MITOS_ERR_ALLOC;
+    dim3 grid((unsigned)((nrows + S::RW - 1) / S::RW), nbatch);
+    ZR2CKV<T, L, S::RW, LOADER, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, loader,
+                                                         out, nrows, N, hs, outb, tw, twN, op);
+    return launch_status();
+}
+
+template <typename T, int N, typename LOADER, typename OP>
+int zr2c_pass_tuned(int variant, LOADER loader, cx<T>* out, long hs, int nbatch, long outb,
+                    OP op, hipStream_t s) {
+    switch (variant) {
+        case 1: return zr2c_pass_v<T, N, 1>(loader, out, hs, nbatch, outb, op, s);
+        case 2: return zr2c_pass_v<T, N, 2>(loader, out, hs, nbatch, outb, op, s);
+        default: return zr2c_pass_v<T, N, 0>(loader, out, hs, nbatch, outb, op, s);
+    }
 }
 
 // ---- composed 3D transforms ---------------------------------------------------

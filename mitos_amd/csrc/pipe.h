@@ -234,6 +234,29 @@ struct CombineK<T, N, Radices<Rs...>> {
     }
 };
 
+// ---- FFT schedule autotuner ------------------------------------------------------------
+// The plain passes of the pipeline have kFftVariants schedules each (radix
+// split, tile width, rows per work-group; see fft_launch.h).  The first
+// pipeline of a (device, N, precision) times every variant on its own grids
+// and keeps the fastest; later pipelines of the same key reuse the choice,
+// so every run in the process uses the same kernels (ensembles and unbatched
+// runs give identical bits).  MITOS_FFT_SCHEDULE=default skips the timing.
+enum TuneKind { TUNE_Y_FWD = 0, TUNE_X_FWD = 1, TUNE_Z_C2R = 2, TUNE_Z_R2C = 3, TUNE_KINDS = 4 };
+
+struct FftSchedule {
+    int v[TUNE_KINDS] = {0, 0, 0, 0};
+    float ms[TUNE_KINDS][kFftVariants] = {};
+};
+
+inline std::mutex& schedule_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+inline std::map<std::tuple<int, int, int>, FftSchedule>& schedule_table() {
+    static std::map<std::tuple<int, int, int>, FftSchedule> t;
+    return t;
+}
+
 // ---- the fused pipeline ------------------------------------------------------------------
 template <typename T>
 struct Pipe {
@@ -270,6 +293,68 @@ struct Pipe {
             return rc;
         }
         return f();
+    }
+
+    FftSchedule sched;
+
+    // time every schedule variant of the plain passes (or reuse the choice
+    // made for this device, N and precision)
+    int autotune(hipStream_t s) {
+        int dev = 0;
+        MITOS_HIP_CHECK(hipGetDevice(&dev));
+        const auto key = std::make_tuple(dev, N, (int)sizeof(T));
+        std::lock_guard<std::mutex> lock(schedule_mutex());
+        auto& table = schedule_table();
+        auto it = table.find(key);
+        if (it != table.end()) {
+            sched = it->second;
+            return MITOS_OK;
+        }
+        FftSchedule sc;
+        const char* env = getenv("MITOS_FFT_SCHEDULE");
+        if (!(env && strcmp(env, "default") == 0)) {
+            hipEvent_t e0, e1;
+            MITOS_HIP_CHECK(hipEventCreate(&e0));
+            MITOS_HIP_CHECK(hipEventCreate(&e1));
+            int rc = with_size(N, [&]<int NN>() -> int {
+                auto time_it = [&](auto&& launch, float* ms) -> int {
+                    MITOS_TRY(launch());  // warm-up
+                    MITOS_HIP_CHECK(hipEventRecord(e0, s));
+                    for (int r = 0; r < 3; r++) MITOS_TRY(launch());
+                    MITOS_HIP_CHECK(hipEventRecord(e1, s));
+                    MITOS_HIP_CHECK(hipEventSynchronize(e1));
+                    MITOS_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+                    *ms /= 3;
+                    return MITOS_OK;
+                };
+                CicLoader<T> ld{cic(), (long)NN, 1.0};
+                for (int v = 0; v < kFftVariants; v++) {
+                    MITOS_TRY(time_it([&] {
+                        return strided_pass_tuned<T, NN, 1, -1>(v, K, K, hs, 1, 0, OpNone(), s);
+                    }, &sc.ms[TUNE_Y_FWD][v]));
+                    MITOS_TRY(time_it([&] {
+                        return strided_pass_tuned<T, NN, 0, -1>(v, K, K, hs, 1, 0, OpNone(), s);
+                    }, &sc.ms[TUNE_X_FWD][v]));
+                    MITOS_TRY(time_it([&] {
+                        return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(
+                            v, b(0), psi(0), hs, NN + MITOS_PSI_PAD, 1, 0, 0, (T)1, OpNone(), s);
+                    }, &sc.ms[TUNE_Z_C2R][v]));
+                    MITOS_TRY(time_it([&] {
+                        return zr2c_pass_tuned<T, NN>(v, ld, K, hs, 1, 0, OpNone(), s);
+                    }, &sc.ms[TUNE_Z_R2C][v]));
+                }
+                return MITOS_OK;
+            });
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            MITOS_TRY(rc);
+            for (int k = 0; k < TUNE_KINDS; k++)
+                for (int v = 1; v < kFftVariants; v++)
+                    if (sc.ms[k][v] < sc.ms[k][sc.v[k]]) sc.v[k] = v;
+        }
+        table[key] = sc;
+        sched = sc;
+        return MITOS_OK;
     }
 
     cx<T>* a(int i) { return A + (size_t)i * csize; }
@@ -404,7 +489,8 @@ struct Pipe {
                 }));
                 // 5. y pass forward of delta2
                 MITOS_TRY(timed(s, "delta2_y_fwd", 2 * G, [&] {
-                    return strided_pass<T, NN, 1, -1>(D, D, hs, 1, 0, OpNone(), s);
+                    return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], D, D, hs, 1, 0,
+                                                            OpNone(), s);
                 }));
             } else {
                 D = K;  // Zel'dovich: the second-order term is multiplied by zero
@@ -449,16 +535,19 @@ struct Pipe {
             // 8. z c2r: displacement grids (src/mitos.c:404-416)
             const T cs = (T)(1.0 / boxvol);
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass<T, NN, decltype(OpNone()), MITOS_PSI_HO>(
-                    b(0), psi(0), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpNone(), s);
+                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(0), psi(0), hs,
+                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            OpNone(), s);
             }));
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass<T, NN, decltype(OpNone()), MITOS_PSI_HO>(
-                    b(1), psi(1), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpNone(), s);
+                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(1), psi(1), hs,
+                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            OpNone(), s);
             }));
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass<T, NN, decltype(OpIkz{dk}), MITOS_PSI_HO>(
-                    b(2), psi(2), hs, NN + MITOS_PSI_PAD, 1, 0, 0, cs, OpIkz{dk}, s);
+                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(2), psi(2), hs,
+                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            OpIkz{dk}, s);
             }));
             mark(4);
             // 9-10. lattice + sequential TSC displacement + wrap + CIC deposit
@@ -508,13 +597,15 @@ struct Pipe {
             // 11. density r2c (overdensity on load), 12. P(k) with CIC window
             CicLoader<T> ld{cic(), (long)NN, ldexp(1.0, -cic_shift())};
             MITOS_TRY(timed(s, "density_z_r2c", 8.0 * np + G, [&] {
-                return zr2c_pass<T, NN>(ld, K, hs, 1, 0, OpNone(), s);
+                return zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, K, hs, 1, 0, OpNone(), s);
             }));
             MITOS_TRY(timed(s, "density_y_fwd", 2 * G, [&] {
-                return strided_pass<T, NN, 1, -1>(K, K, hs, 1, 0, OpNone(), s);
+                return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, K, hs, 1, 0,
+                                                        OpNone(), s);
             }));
             MITOS_TRY(timed(s, "density_x_fwd", 2 * G, [&] {
-                return strided_pass<T, NN, 0, -1>(K, K, hs, 1, 0, OpNone(), s);
+                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], K, K, hs, 1, 0,
+                                                        OpNone(), s);
             }));
             const double* w2 = window_table(NN, L, 2);
             if (!w2) return MITOS_ERR_ALLOC;

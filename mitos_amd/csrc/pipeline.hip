@@ -51,9 +51,11 @@ extern "C" int mitos_ic_create(const struct mitos_ic_config* cfg, mitos_ic_conte
     if (cfg->precision == MITOS_FP32) {
         c->pf = new Pipe<float>();
         rc = c->pf->create(cfg->N);
+        if (rc == MITOS_OK) rc = c->pf->autotune(c->stream);
     } else {
         c->pd = new Pipe<double>();
         rc = c->pd->create(cfg->N);
+        if (rc == MITOS_OK) rc = c->pd->autotune(c->stream);
     }
     if (rc != MITOS_OK) {
         mitos_ic_destroy(c);
@@ -132,6 +134,29 @@ extern "C" int mitos_ic_run(mitos_ic_context* c, const struct mitos_ic_config* c
         hipEventElapsedTime(&tot, ev[0], ev[7]);
         res->stage_ms[7] = tot;
     }
+    return MITOS_OK;
+}
+
+extern "C" int mitos_ic_schedule(mitos_ic_context* c, int kind, float* ms) {
+    if (!c || kind < 0 || kind >= TUNE_KINDS) return -1;
+    const FftSchedule& sc = c->pf ? c->pf->sched : c->pd->sched;
+    if (ms)
+        for (int v = 0; v < kFftVariants; v++) ms[v] = sc.ms[kind][v];
+    return sc.v[kind];
+}
+
+extern "C" int mitos_set_fft_schedule(int N, int precision, const int* variants) {
+    if (!variants || !size_supported(N) || (precision != MITOS_FP32 && precision != MITOS_FP64))
+        return set_last_error(MITOS_ERR_ARG);
+    FftSchedule sc;
+    for (int k = 0; k < TUNE_KINDS; k++) {
+        if (variants[k] < 0 || variants[k] >= kFftVariants) return set_last_error(MITOS_ERR_ARG);
+        sc.v[k] = variants[k];
+    }
+    int dev = 0;
+    MITOS_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(schedule_mutex());
+    schedule_table()[std::make_tuple(dev, N, precision == MITOS_FP32 ? 4 : 8)] = sc;
     return MITOS_OK;
 }
 

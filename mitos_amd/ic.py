@@ -211,6 +211,15 @@ class IC:
         cfg = cfg or self.cfg
         check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
 
+    def schedules(self) -> dict:
+        """Autotuned FFT schedule per pass kind: (variant, [ms per variant])."""
+        out = {}
+        for kind, name in enumerate(("y_fwd", "x_fwd", "z_c2r", "z_r2c")):
+            ms = (ctypes.c_float * 3)()
+            v = lib().mitos_ic_schedule(self.h, kind, ms)
+            out[name] = (v, [float(x) for x in ms])
+        return out
+
     def profile(self, enable: bool = True) -> None:
         check(lib().mitos_ic_profile(self.h, int(enable)), "mitos_ic_profile")
 
@@ -274,6 +283,13 @@ class IC:
             self.close()
         except Exception:
             pass
+
+
+def set_fft_schedule(N: int, precision: int, variants) -> None:
+    """Pin the FFT schedule variants (y fwd, x fwd, z c2r, z r2c) of later
+    contexts of this size and precision (mitos_set_fft_schedule)."""
+    arr = (ctypes.c_int * 4)(*variants)
+    check(lib().mitos_set_fft_schedule(N, precision, arr), "mitos_set_fft_schedule")
 
 
 def run_reference_order(cfg: ICConfig, positions: bool = False) -> dict:

@@ -94,3 +94,26 @@ def test_fused_repeatable(gpu):
     ok = a["obs"] > 0
     assert np.array_equal(a["P"][ok], b["P"][ok])
     ic.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_every_fft_schedule_variant(gpu, oracle, variant):
+    # each schedule the autotuner can pick gives the oracle's P(k) and psi
+    N, L, seed, bins = 32, 128.0, 11, 16
+    print("case schedule variant", variant, flush=True)
+    ref = _oracle_run(oracle, N, L, seed, bins)
+    gpu.set_fft_schedule(N, gpu.MITOS_FP64, [variant] * 4)
+    ic = gpu.IC(gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=gpu.MITOS_FP64))
+    assert all(v == variant for v, _ in ic.schedules().values())
+    out = ic.run()
+    err = _pk_rel(out["P"], ref["P"], ref["obs"])
+    assert err.max() < 1e-10
+    dens = ic.density().cpu().numpy()
+    de = np.linalg.norm(dens - ref["density"]) / np.linalg.norm(ref["density"])
+    assert de < 1e-6
+    ic.close()
+    gpu.set_fft_schedule(N, gpu.MITOS_FP32, [variant] * 4)
+    ic = gpu.IC(gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=gpu.MITOS_FP32))
+    out = ic.run()
+    assert _pk_rel(out["P"], ref["P"], ref["obs"]).max() < 1e-4
+    ic.close()
