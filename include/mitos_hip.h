@@ -328,6 +328,23 @@ int mitos_ic_profile_entry(mitos_ic_context *ctx, int i, const char **name, floa
 int mitos_ic_run_reference_order(const struct mitos_ic_config *cfg,
                                  struct mitos_ic_result *res, float *X, float *Y, float *Z);
 
+/* ---- adjoint: gradient of a P(k) or density loss w.r.t. the white noise -- */
+/* noise / grad_noise: device, dense N x N x (N/2+1) complex of the context's
+ * real type (unit-variance white noise before the spectrum amplitude, in the
+ * layout of generate_complex_grf's output with the row stride N/2+1).
+ * _forward runs the pipeline from this noise (lattice TSC stencils applied in
+ * k-space for both precisions, the form the adjoint differentiates) and
+ * returns P(k) like mitos_ic_run.  _backward recomputes that forward run and
+ * writes dL/dnoise (d/dRe + i d/dIm per mode) given dL/dP (host, cfg.bins
+ * values; bins with no modes are ignored) and/or dL/ddelta (device, dense N^3
+ * of the real type), either may be NULL.  The phi_,ij grids are recomputed,
+ * not stored. */
+int mitos_ic_noise_forward(mitos_ic_context *ctx, const struct mitos_ic_config *cfg,
+                           const void *noise, struct mitos_ic_result *res);
+int mitos_ic_noise_backward(mitos_ic_context *ctx, const struct mitos_ic_config *cfg,
+                            const void *noise, const double *dL_dP, const void *dL_ddelta,
+                            void *grad_noise);
+
 /* ======================================================================== */
 /* Part 4: ensembles -- n realizations of uneven size (any mix of N, boxlen, */
 /* seed, n_s, bins, precision) captured once as one HIP graph and replayed.  */

@@ -165,6 +165,10 @@ SIGNATURES = [
                                               ctypes.POINTER(ctypes.c_double)]),
     ("mitos_ic_run_reference_order", ctypes.c_int, [ctypes.POINTER(IcConfig),
                                                     ctypes.POINTER(IcResult), vp, vp, vp]),
+    ("mitos_ic_noise_forward", ctypes.c_int, [vp, ctypes.POINTER(IcConfig), vp,
+                                              ctypes.POINTER(IcResult)]),
+    ("mitos_ic_noise_backward", ctypes.c_int, [vp, ctypes.POINTER(IcConfig), vp, c_double_p, vp,
+                                               vp]),
     ("mitos_ensemble_create", ctypes.c_int, [ctypes.POINTER(IcConfig), ctypes.c_int,
                                              ctypes.POINTER(vp)]),
     ("mitos_ensemble_run", ctypes.c_int, [vp]),

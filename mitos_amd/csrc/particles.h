@@ -295,10 +295,37 @@ __global__ void __launch_bounds__(NT)
 // per axis), so the moves read 1 + 3 + 9 values instead of 3 x 27.
 // Work-groups walk strips of TX x-planes row by row in y, so the x taps of
 // neighbouring rows stay in L2.
-template <int NT, int TX>
+// TSC weights in arithmetic M (float: tsc_axis_near; double: the same
+// closed form in double) and, for the adjoint, their derivatives in X.
+__device__ __forceinline__ void tsc_axis_near(double X, int& c, double w[3]) {
+    const double cf = floor(X + 0.5);
+    const double u = X - cf;
+    c = (int)cf;
+    const double a = 0.5 - u, b = 0.5 + u;
+    w[0] = 0.5 * (a * a);
+    w[1] = 0.75 - u * u;
+    w[2] = 0.5 * (b * b);
+}
+template <typename M>
+__device__ __forceinline__ void tsc_axis_near_d(M X, int& c, M w[3], M dw[3]) {
+    const M cf = floor(X + (M)0.5);
+    const M u = X - cf;
+    c = (int)cf;
+    const M a = (M)0.5 - u, b = (M)0.5 + u;
+    w[0] = (M)0.5 * (a * a);
+    w[1] = (M)0.75 - u * u;
+    w[2] = (M)0.5 * (b * b);
+    dw[0] = -a;
+    dw[1] = (M)-2 * u;
+    dw[2] = b;
+}
+
+// T: grid and arithmetic type (float in the FP32 pipeline; double when the
+// FP64 pipeline runs in this form for the adjoint).
+template <typename T, int NT, int TX>
 __global__ void __launch_bounds__(NT)
-    k_displace_lattice_s(const float* __restrict__ sx, const float* __restrict__ sy,
-                         const float* __restrict__ sz, unsigned rs, int N, int nch, float Lf,
+    k_displace_lattice_s(const T* __restrict__ sx, const T* __restrict__ sy,
+                         const T* __restrict__ sz, unsigned rs, int N, int nch, T Lf,
                          float spacing, double boxlen, float* __restrict__ X,
                          float* __restrict__ Y, float* __restrict__ Z) {
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
@@ -309,34 +336,34 @@ __global__ void __launch_bounds__(NT)
     const int qy = (int)(r2 % (unsigned)N);
     const int qx = (int)(r2 / (unsigned)N) * TX + (int)xi;
     const float invN = 1.0f / N;
-    const float Nf = (float)N;
+    const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
     const unsigned yz = (unsigned)qy * rs + MITOS_PSI_HO + qz;
     float x = qx * spacing, y = qy * spacing, z = qz * spacing;  // src/particle.c:107-109
     // x move: the stencilled x displacement at the site
-    x = x - sx[(unsigned)qx * plane + yz];
+    x = (float)((T)x - sx[(unsigned)qx * plane + yz]);
     // y move: TSC along x at (x', q_y, q_z)
     int cx;
-    float wx[3];
-    tsc_axis_near(x * Nf / Lf, cx, wx);
+    T wx[3];
+    tsc_axis_near((T)x * Nf / Lf, cx, wx);
     unsigned ox[3];
     tap_rows(wrapf(cx, N, invN), N, plane, ox);
-    y = y - fmaf(wx[2], sy[ox[2] + yz], fmaf(wx[1], sy[ox[1] + yz], wx[0] * sy[ox[0] + yz]));
+    y = (float)((T)y - fma(wx[2], sy[ox[2] + yz], fma(wx[1], sy[ox[1] + yz], wx[0] * sy[ox[0] + yz])));
     // z move: TSC along x and y at (x', y', q_z)
     int cy;
-    float wy[3];
-    tsc_axis_near(y * Nf / Lf, cy, wy);
+    T wy[3];
+    tsc_axis_near((T)y * Nf / Lf, cy, wy);
     unsigned oy[3];
     tap_rows(wrapf(cy, N, invN), N, rs, oy);
     const unsigned zo = MITOS_PSI_HO + qz;
-    float d = 0;
+    T d = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const unsigned o = ox[a] + zo;
-        const float r = fmaf(wy[2], sz[o + oy[2]], fmaf(wy[1], sz[o + oy[1]], wy[0] * sz[o + oy[0]]));
-        d = fmaf(wx[a], r, d);
+        const T r = fma(wy[2], sz[o + oy[2]], fma(wy[1], sz[o + oy[1]], wy[0] * sz[o + oy[0]]));
+        d = fma(wx[a], r, d);
     }
-    z = z - d;
+    z = (float)((T)z - d);
     const long long p = ((long long)qx * N + qy) * N + qz;
     X[p] = (float)fwrap_fast(x, boxlen);
     Y[p] = (float)fwrap_fast(y, boxlen);
@@ -458,6 +485,105 @@ __global__ void __launch_bounds__(256)
     unsigned long long w[8];
     cic_weights(dx, dy, dz, w);
     cic_global(grid, N, iX, iY, iZ, w);
+}
+
+// The three stencil-form moves of one lattice particle in arithmetic T,
+// without rounding the positions to float (used by the adjoint and by the
+// forward run it differentiates).  Outputs the moved, unwrapped position.
+template <typename T>
+__device__ __forceinline__ void moves_s(const T* __restrict__ sx, const T* __restrict__ sy,
+                                        const T* __restrict__ sz, unsigned rs, int N, T sN,
+                                        float spacing, int qx, int qy, int qz, T& x, T& y,
+                                        T& z) {
+    const float invN = 1.0f / N;
+    const unsigned plane = (unsigned)N * rs;
+    const unsigned yz = (unsigned)qy * rs + MITOS_PSI_HO + qz;
+    x = (T)(qx * spacing) - sx[(unsigned)qx * plane + yz];
+    y = (T)(qy * spacing);
+    z = (T)(qz * spacing);
+    int cx;
+    T wx[3];
+    tsc_axis_near(x * sN, cx, wx);
+    unsigned ox[3];
+    tap_rows(wrapf(cx, N, invN), N, plane, ox);
+    y -= wx[0] * sy[ox[0] + yz] + wx[1] * sy[ox[1] + yz] + wx[2] * sy[ox[2] + yz];
+    int cy;
+    T wy[3];
+    tsc_axis_near(y * sN, cy, wy);
+    unsigned oy[3];
+    tap_rows(wrapf(cy, N, invN), N, rs, oy);
+    const unsigned zo = MITOS_PSI_HO + qz;
+    T d = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) d += wx[a] * wy[b] * sz[ox[a] + oy[b] + zo];
+    z -= d;
+}
+
+// periodic wrap of a coordinate in T (exact subtractions for |x| < L)
+template <typename T>
+__device__ __forceinline__ T wrap_pos(T x, T L) {
+    if (x < 0) x += L;
+    if (x >= L) x -= L;
+    if (x < 0 || x >= L) x = x - L * floor(x / L);
+    return x;
+}
+
+// CIC cell and weights of a position in T (the cell offsets of
+// analyse/src/mitos_read.c:285-320 without the float position)
+template <typename T>
+__device__ __forceinline__ void cic_cell_t(T x, T icell, int N, int& i, T& d) {
+    const T X = x * icell;
+    T f = floor(X);
+    i = (int)f;
+    d = X - f;
+    if (i >= N) {
+        i -= N;
+    }
+}
+
+// Adjoint-mode particles: moves in T, positions kept in T for the deposit
+// (their float copies go to X/Y/Z), fixed-point weights 2^-shift with global
+// atomics (order independent).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_move_deposit_s(const T* __restrict__ sx, const T* __restrict__ sy,
+                     const T* __restrict__ sz, unsigned rs, int N, double boxlen, float spacing,
+                     float* X, float* Y, float* Z, unsigned long long* grid, int shift) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)N * N * N;
+    if (p >= n) return;
+    const int qz = (int)(p % N), qy = (int)((p / N) % N), qx = (int)(p / ((long long)N * N));
+    const T L = (T)boxlen, sN = (T)(N / boxlen);
+    T x, y, z;
+    moves_s(sx, sy, sz, rs, N, sN, spacing, qx, qy, qz, x, y, z);
+    x = wrap_pos(x, L);
+    y = wrap_pos(y, L);
+    z = wrap_pos(z, L);
+    X[p] = (float)x;
+    Y[p] = (float)y;
+    Z[p] = (float)z;
+    int iX, iY, iZ;
+    T dx, dy, dz;
+    cic_cell_t(x, sN, N, iX, dx);
+    cic_cell_t(y, sN, N, iY, dy);
+    cic_cell_t(z, sN, N, iZ, dz);
+    const T S = (T)ldexp(1.0, shift);
+    const T tx = 1 - dx, ty = 1 - dy, tz = 1 - dz;
+    const T w[8] = {tx * ty * tz, dx * ty * tz, tx * dy * tz, tx * ty * dz,
+                    dx * dy * tz, dx * ty * dz, tx * dy * dz, dx * dy * dz};
+    unsigned long long q[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        // float weights convert through 32 bits (w * 2^31 + 1/2 < 2^32), as
+        // cic_weights(float) does
+        if constexpr (sizeof(T) == 4)
+            q[c] = (unsigned)fmaf(w[c], S, 0.5f);
+        else
+            q[c] = (unsigned long long)(w[c] * S + (T)0.5);
+    }
+    cic_global(grid, N, iX, iY, iZ, q);
 }
 
 // CIC deposit (analyse/src/mitos_read.c:281-334) of particles stored in

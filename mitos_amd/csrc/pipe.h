@@ -156,6 +156,24 @@ inline const double* phi1_table(int N, double boxlen, double A, double n_s, doub
     return d;
 }
 
+// K = noise * sqrt(L^3/2) * amp[n^2] (the scaling of src/grf.c:33 and the
+// phi1 table), DC = 0; noise dense N x N x (N/2+1)
+template <typename T>
+__global__ void k_noise_in(const cx<T>* __restrict__ noise, cx<T>* K, long hs, int N,
+                           double boxlen, const double* __restrict__ amp) {
+    const int H = N / 2 + 1;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)N * N * H) return;
+    const int z = (int)(i % H);
+    const long row = i / H;
+    const int x = (int)(row / N), y = (int)(row % N);
+    const int nx = freq_index(x, N), ny = freq_index(y, N);
+    const long n2 = (long)nx * nx + (long)ny * ny + (long)z * z;
+    const double f = sqrt(boxlen * boxlen * boxlen / 2) * amp[n2];
+    const cx<T> v = noise[i];
+    K[row * hs + z] = n2 ? mk<T>((T)((double)v.re * f), (T)((double)v.im * f)) : mk<T>(0, 0);
+}
+
 template <typename T>
 __global__ void k_fill_u64(unsigned long long* p, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,6 +314,7 @@ struct Pipe {
     }
 
     FftSchedule sched;
+    bool psi_stencilled = false;  // the displacement grids hold the stencil form
 
     // time every schedule variant of the plain passes (or reuse the choice
     // made for this device, N and precision)
@@ -422,9 +441,16 @@ struct Pipe {
     // P(k) sums and counts instead of pk_sums/pk_obs.  No allocation and no
     // host synchronisation happen here once the tables of cfg exist, so the
     // call can be captured into a graph.
+    // noise_in: unit-variance complex white noise (dense N x N x (N/2+1) of cx<T>)
+    // used instead of the seed's stream.  stencil: -1 as the precision
+    // (FP32 applies the lattice TSC stencils in k-space, FP64 gathers 27 taps),
+    // 1 forces the stencil form (the form the adjoint differentiates).
     int enqueue(const mitos_ic_config& cfg, hipStream_t s, bool timing,
                 const double* amp_tab = nullptr, double* pk_dst = nullptr,
-                long long* obs_dst = nullptr) {
+                long long* obs_dst = nullptr, const cx<T>* noise_in = nullptr,
+                int stencil = -1) {
+        const bool STENCIL = stencil < 0 ? sizeof(T) == 4 : stencil != 0;
+        psi_stencilled = STENCIL;
         const double L = cfg.boxlen;
         const double dk = 2 * M_PI / L;
         const double boxvol = L * L * L;
@@ -448,6 +474,11 @@ struct Pipe {
         if (!amp) return MITOS_ERR_ALLOC;
         // FP64 keeps correctly rounded Box-Muller; FP32 rounds every value to float
         MITOS_TRY(timed(s, "grf_noise", G, [&] {
+            if (noise_in) {
+                hipLaunchKernelGGL((k_noise_in<T>), dim3(nblk((long)N * N * (N / 2 + 1))),
+                                   dim3(256), 0, s, noise_in, K, hs, N, L, amp);
+                return launch_status();
+            }
             return launch_grf<T, AmpTable, sizeof(T) == 8>(K, hs, N, L, 1, 0, J, AmpTable{amp}, s);
         }));
         MITOS_TRY(timed(s, "hermitian_planes", 0.0,
@@ -505,8 +536,8 @@ struct Pipe {
             cp.f2 = cfg.run2lpt ? cfg.f2 : 0.0;
             cp.win = window_table(NN, L, 3);
             if (!cp.win) return MITOS_ERR_ALLOC;
-            // FP32 applies the lattice TSC stencils in k-space (k_displace_lattice_s)
-            constexpr bool STENCIL = sizeof(T) == 4;
+            // the stencil form applies the lattice TSC stencils in k-space
+            // (k_displace_lattice_s)
             cp.smooth = nullptr;
             if (STENCIL) {
                 cp.smooth = smooth_table<double>(NN);
@@ -526,11 +557,14 @@ struct Pipe {
                 p7.smooth = smooth_table<T>(NN);
                 if (!p7.smooth) return MITOS_ERR_ALLOC;
             }
-            constexpr int E1CODE = STENCIL ? MITOS_OUT(0, 3) : MITOS_OUT(0, 1);
             MITOS_TRY(timed(s, "displacement_y_2to3", 5 * G, [&] {
+                if (STENCIL)
+                    return MultiK<T, NN, 1, 1, 2,
+                                  OutCodes<MITOS_OUT(0, 3), MITOS_OUT(0, 0),
+                                           MITOS_OUT(1, 0)>>::go(p7, hs, dk, s);
                 return MultiK<T, NN, 1, 1, 2,
-                              OutCodes<E1CODE, MITOS_OUT(0, 0), MITOS_OUT(1, 0)>>::go(p7, hs, dk,
-                                                                                     s);
+                              OutCodes<MITOS_OUT(0, 1), MITOS_OUT(0, 0), MITOS_OUT(1, 0)>>::go(
+                    p7, hs, dk, s);
             }));
             // 8. z c2r: displacement grids (src/mitos.c:404-416)
             const T cs = (T)(1.0 / boxvol);
@@ -560,18 +594,25 @@ struct Pipe {
             mark(5);
             // FP32 moves the particles in float arithmetic, FP64 in double
             typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
-            {
+            if (noise_in) {
+                // the forward of the adjoint: positions stay in T up to the deposit
+                MITOS_TRY(timed(s, "move_deposit", 3 * Rg + 12.0 * np + 8.0 * np, [&] {
+                    hipLaunchKernelGGL((k_move_deposit_s<T>), dim3(nblk(np)), dim3(256), 0, s,
+                                       psi(0), psi(1), psi(2), (unsigned)(NN + MITOS_PSI_PAD),
+                                       NN, L, spacing, X, Y, Z, cic(), cic_shift());
+                    return launch_status();
+                }));
+            } else {
                 // reads 3 displacement grids, writes 3 float coordinates
                 MITOS_TRY(timed(s, "displace_tsc", 3 * Rg + 12.0 * np, [&] {
                     constexpr int NT = NN >= 256 ? 256 : 64;
                     const int nch = (NN + NT - 1) / NT;
                     const dim3 grid((unsigned)((long)NN * NN * nch));
                     const unsigned rs = (unsigned)(NN + MITOS_PSI_PAD);
-                    if constexpr (STENCIL) {
+                    if (STENCIL) {
                         constexpr int TX = NN % 16 == 0 ? 16 : 8;
-                        hipLaunchKernelGGL((k_displace_lattice_s<NT, TX>), grid, dim3(NT), 0, s,
-                                           (const float*)psi(0), (const float*)psi(1),
-                                           (const float*)psi(2), rs, NN, nch, (float)L, spacing,
+                        hipLaunchKernelGGL((k_displace_lattice_s<T, NT, TX>), grid, dim3(NT), 0,
+                                           s, psi(0), psi(1), psi(2), rs, NN, nch, (T)L, spacing,
                                            L, X, Y, Z);
                     } else {
                         hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,

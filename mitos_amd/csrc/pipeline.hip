@@ -87,6 +87,18 @@ static int check_cfg(mitos_ic_context* c, const struct mitos_ic_config* cfg) {
     return MITOS_OK;
 }
 
+namespace mitos {
+Pipe<float>* ctx_pipe_f(mitos_ic_context* c) { return c->pf; }
+Pipe<double>* ctx_pipe_d(mitos_ic_context* c) { return c->pd; }
+hipStream_t ctx_stream(mitos_ic_context* c) { return c->stream; }
+int ctx_use_cfg(mitos_ic_context* c, const mitos_ic_config* cfg) {
+    MITOS_TRY(check_cfg(c, cfg));
+    c->bins = cfg->bins;
+    c->boxlen = cfg->boxlen;
+    return MITOS_OK;
+}
+}  // namespace mitos
+
 extern "C" int mitos_ic_enqueue(mitos_ic_context* c, const struct mitos_ic_config* cfg,
                                 void* stream) {
     MITOS_TRY(set_last_error(check_cfg(c, cfg)));
@@ -233,7 +245,8 @@ __global__ void k_copy_psi(const T* src, long src_stride, T* dst, long n, int N)
 // FP32: the grids hold S_xyz psi_x, S_yz psi_y, S_z psi_z (see
 // k_displace_lattice_s); psi is recovered by r2c, division by the stencils and
 // c2r, in the pipeline's scratch grids (K and A, free after a run)
-static int unstencil_psi(Pipe<float>* P, int N, float* out, hipStream_t s) {
+template <typename T>
+static int unstencil_psi(Pipe<T>* P, int N, T* out, hipStream_t s) {
     const double* S = smooth_table<double>(N);
     if (!S) return MITOS_ERR_ALLOC;
     const GridLayout gin{N, P->hs, (long)N + MITOS_PSI_PAD};
@@ -241,10 +254,10 @@ static int unstencil_psi(Pipe<float>* P, int N, float* out, hipStream_t s) {
     const double norm = 1.0 / ((double)N * N * N);
     const int masks[3] = {7, 6, 4};
     for (int i = 0; i < 3; i++) {
-        RealRowLoader<float> ld{P->psi(i) + MITOS_PSI_HO, gin.rs, 0};
-        MITOS_TRY((r2c_3d<float>(ld, P->K, gin, 1, 0, OpNone(), OpNone(), OpNone(), s)));
-        MITOS_TRY((c2r_3d<float>(P->K, P->a(0), out + (size_t)i * N * N * N, gout, 1, 0, 0, 1.0f,
-                                 OpUnstencil{S, masks[i], norm}, OpNone(), OpNone(), s)));
+        RealRowLoader<T> ld{P->psi(i) + MITOS_PSI_HO, gin.rs, 0};
+        MITOS_TRY((r2c_3d<T>(ld, P->K, gin, 1, 0, OpNone(), OpNone(), OpNone(), s)));
+        MITOS_TRY((c2r_3d<T>(P->K, P->a(0), out + (size_t)i * N * N * N, gout, 1, 0, 0, (T)1,
+                             OpUnstencil{S, masks[i], norm}, OpNone(), OpNone(), s)));
     }
     return MITOS_OK;
 }
@@ -252,8 +265,13 @@ static int unstencil_psi(Pipe<float>* P, int N, float* out, hipStream_t s) {
 extern "C" int mitos_ic_displacements(mitos_ic_context* c, void* psi) {
     if (!c || !psi) return set_last_error(MITOS_ERR_ARG);
     const long n = (long)c->N * c->N * c->N;
-    if (c->pf)
+    if (c->pf && c->pf->psi_stencilled)
         MITOS_TRY(set_last_error(unstencil_psi(c->pf, c->N, (float*)psi, c->stream)));
+    else if (c->pd && c->pd->psi_stencilled)
+        MITOS_TRY(set_last_error(unstencil_psi(c->pd, c->N, (double*)psi, c->stream)));
+    else if (c->pf)
+        hipLaunchKernelGGL((k_copy_psi<float>), dim3(nblk(3 * n)), dim3(256), 0, c->stream,
+                           c->pf->psi(0), 2 * c->pf->csize, (float*)psi, n, c->N);
     else
         hipLaunchKernelGGL((k_copy_psi<double>), dim3(nblk(3 * n)), dim3(256), 0, c->stream,
                            c->pd->psi(0), 2 * c->pd->csize, (double*)psi, n, c->N);

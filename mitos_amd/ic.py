@@ -211,6 +211,51 @@ class IC:
         cfg = cfg or self.cfg
         check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
 
+    def noise_forward(self, cfg: ICConfig, noise) -> dict:
+        """P(k) of the pipeline run from `noise` (torch CUDA complex tensor of
+        shape (N, N, N/2+1), complex64 for FP32 contexts, complex128 for FP64)."""
+        self._check_noise(cfg, noise)
+        bins = cfg.bins
+        k = np.empty(bins)
+        P = np.empty(bins)
+        obs = np.empty(bins, np.int64)
+        res = _lib.IcResult(k.ctypes.data_as(_lib.c_double_p), P.ctypes.data_as(_lib.c_double_p),
+                            obs.ctypes.data_as(_lib.c_ll_p), 0.0)
+        _sync()
+        check(lib().mitos_ic_noise_forward(self.h, ctypes.byref(cfg.c()), _ptr(noise),
+                                           ctypes.byref(res)), "mitos_ic_noise_forward")
+        return {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude}
+
+    def noise_backward(self, cfg: ICConfig, noise, dL_dP=None, dL_ddelta=None):
+        """dL/dnoise (same shape and dtype as `noise`) for a loss with gradient
+        dL_dP (bins values) and/or dL_ddelta (CUDA tensor (N, N, N))."""
+        import torch
+        self._check_noise(cfg, noise)
+        grad = torch.empty_like(noise)
+        gp = None
+        if dL_dP is not None:
+            gp = np.ascontiguousarray(np.nan_to_num(np.asarray(dL_dP, np.float64)))
+            assert gp.size == cfg.bins
+        gd = None
+        if dL_ddelta is not None:
+            gd = dL_ddelta.contiguous().to(self._real())
+            assert gd.numel() == cfg.N ** 3 and gd.is_cuda
+        _sync()
+        check(lib().mitos_ic_noise_backward(
+            self.h, ctypes.byref(cfg.c()), _ptr(noise),
+            gp.ctypes.data_as(_lib.c_double_p) if gp is not None else None,
+            _ptr(gd) if gd is not None else None, _ptr(grad)), "mitos_ic_noise_backward")
+        return grad
+
+    def _check_noise(self, cfg, noise):
+        import torch
+        N = cfg.N
+        want = torch.complex64 if self.cfg.precision == MITOS_FP32 else torch.complex128
+        if (tuple(noise.shape) != (N, N, N // 2 + 1) or noise.dtype != want or not noise.is_cuda
+                or not noise.is_contiguous()):
+            raise ValueError(f"noise must be a contiguous CUDA {want} tensor of shape "
+                             f"({N}, {N}, {N // 2 + 1})")
+
     def schedules(self) -> dict:
         """Autotuned FFT schedule per pass kind: (variant, [ms per variant])."""
         out = {}
@@ -373,3 +418,16 @@ def ensemble_configs(scale: int = 1) -> list:
         out.append(ICConfig(N=N, boxlen=L, seed=i + 1, n_s=0.94 + 0.05 * i / 95, bins=32,
                             run2lpt=1, precision=MITOS_FP32))
     return out
+
+
+def white_noise(N: int, seed: int, precision: int = MITOS_FP64):
+    """Unit-variance complex white noise of the seed's stream (the draws of
+    generate_complex_grf, before any amplitude), as a CUDA tensor of shape
+    (N, N, N/2+1) for the noise_* entry points."""
+    import torch
+    g = Grid(N, 2.0 ** (1.0 / 3.0))  # sqrt(L^3/2) = 1
+    st = rand_uint64_init(seed)
+    check(generate_complex_grf(g, st), "generate_complex_grf")
+    out = torch.from_numpy(g.fbox()).cuda()
+    g.free()
+    return out.to(torch.complex64) if precision == MITOS_FP32 else out

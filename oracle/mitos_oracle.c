@@ -61,6 +61,26 @@ void orc_rng_draws(uint64_t seed, uint64_t skip, uint64_t n, uint64_t *out) {
     for (uint64_t i = 0; i < n; i++) out[i] = orc_rng_next(&st);
 }
 
+/* draws offs[i] .. offs[i]+k-1 for n offsets in ascending order, in one pass */
+void orc_rng_draws_at(uint64_t seed, const uint64_t *offs, uint64_t n, uint64_t k,
+                      uint64_t *out) {
+    orc_rng st = orc_rng_init(seed);
+    uint64_t pos = 0, i = 0;
+    uint64_t *ring = (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1));
+    while (i < n) {
+        while (pos < offs[i]) {
+            orc_rng_next(&st);
+            pos++;
+        }
+        /* draws offs[i] .. offs[i]+k-1; later offsets may overlap this window */
+        orc_rng cp = st;
+        for (uint64_t j = 0; j < k; j++) ring[j] = orc_rng_next(&cp);
+        for (uint64_t j = 0; j < k; j++) out[i * k + j] = ring[j];
+        i++;
+    }
+    free(ring);
+}
+
 /* generator state after n draws from the seed's start */
 void orc_rng_state_after(uint64_t seed, uint64_t n, uint64_t out[4]) {
     orc_rng st = orc_rng_init(seed);

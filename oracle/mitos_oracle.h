@@ -29,6 +29,8 @@ orc_rng orc_rng_init(uint64_t seed);
 uint64_t orc_rng_next(orc_rng *st);
 /* n raw draws after skipping `skip` draws from the seed's start */
 void orc_rng_draws(uint64_t seed, uint64_t skip, uint64_t n, uint64_t *out);
+void orc_rng_draws_at(uint64_t seed, const uint64_t *offs, uint64_t n, uint64_t k,
+                      uint64_t *out);
 void orc_rng_state_after(uint64_t seed, uint64_t n, uint64_t out[4]);
 /* src/random.c:34-49 */
 double orc_sample_norm(orc_rng *st);

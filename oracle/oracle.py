@@ -68,6 +68,8 @@ def lib():
         L.orc_rng_init.argtypes = [ctypes.c_uint64]
         L.orc_rng_draws.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, c_u64_p]
         L.orc_rng_state_after.argtypes = [ctypes.c_uint64, ctypes.c_uint64, c_u64_p]
+        L.orc_rng_draws_at.argtypes = [ctypes.c_uint64, c_u64_p, ctypes.c_uint64,
+                                       ctypes.c_uint64, c_u64_p]
         L.orc_sample_norms.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                        c_double_p]
         L.orc_generate_complex_grf.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRng)]
@@ -130,6 +132,19 @@ def set_threads(n: int) -> int:
 def draws(seed: int, skip: int, n: int) -> np.ndarray:
     out = np.empty(n, dtype=np.uint64)
     lib().orc_rng_draws(seed, skip, n, _p(out, c_u64_p))
+    return out
+
+
+def draws_at(seed: int, offsets, k: int) -> np.ndarray:
+    """k draws at each of the given draw offsets (one pass over the stream);
+    returns shape (len(offsets), k) in the order given"""
+    offs = np.asarray(offsets, dtype=np.uint64)
+    order = np.argsort(offs, kind="stable")
+    srt = np.ascontiguousarray(offs[order])
+    out = np.empty((len(offs), k), dtype=np.uint64)
+    tmp = np.empty((len(offs), k), dtype=np.uint64)
+    lib().orc_rng_draws_at(seed, _p(srt, c_u64_p), len(offs), k, _p(tmp, c_u64_p))
+    out[order] = tmp
     return out
 
 
