@@ -157,6 +157,8 @@ SIGNATURES = [
     ("mitos_ic_ids", ctypes.c_int, [vp, vp]),
     ("mitos_ic_destroy", ctypes.c_int, [vp]),
     ("mitos_ic_schedule", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    ("mitos_ic_fft_timing", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
     ("mitos_set_fft_schedule", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
     ("mitos_ic_profile", ctypes.c_int, [vp, ctypes.c_int]),

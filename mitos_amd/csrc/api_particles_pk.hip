@@ -86,6 +86,64 @@ const int* pk_bin_edges(int N, double boxlen, int bins, int* err) {
     return d;
 }
 
+// multiplicity-weighted count of the half-grid modes with each integer n^2
+static __global__ void k_n2_hist(int N, unsigned int* hist) {
+    const int H = N / 2 + 1;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)N * N * H) return;
+    const int z = (int)(i % H);
+    const long row = i / H;
+    const int nx = freq_index((int)(row / N), N), ny = freq_index((int)(row % N), N);
+    const long n2 = (long)nx * nx + (long)ny * ny + (long)z * z;
+    if (n2 == 0) return;
+    atomicAdd(hist + n2, (z == 0 || z == N / 2) ? 1u : 2u);
+}
+
+int pk_geometry(int N, double boxlen, int bins, PkGeometry* out) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, double, int>, PkGeometry> cache;
+    int dev = 0;
+    MITOS_HIP_CHECK(hipGetDevice(&dev));
+    auto key = std::make_tuple(dev, N, boxlen, bins);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        *out = it->second;
+        return MITOS_OK;
+    }
+    const long n2max = 3L * (N / 2) * (N / 2);
+    std::vector<int> lut(n2max + 1);
+    MITOS_TRY(mitos_pk_bin_table(N, boxlen, bins, lut.data()));
+    unsigned int* dh = nullptr;
+    MITOS_HIP_CHECK(hipMalloc(&dh, sizeof(unsigned int) * (n2max + 1)));
+    MITOS_HIP_CHECK(hipMemset(dh, 0, sizeof(unsigned int) * (n2max + 1)));
+    const long nmodes = (long)N * N * (N / 2 + 1);
+    hipLaunchKernelGGL(k_n2_hist, dim3((unsigned)((nmodes + 255) / 256)), dim3(256), 0, 0, N, dh);
+    std::vector<unsigned int> hist(n2max + 1);
+    const hipError_t e = hipMemcpy(hist.data(), dh, sizeof(unsigned int) * (n2max + 1),
+                                   hipMemcpyDeviceToHost);
+    hipFree(dh);
+    MITOS_HIP_CHECK(e);
+    std::vector<double> ksum(bins, 0.0);
+    std::vector<long long> obs(bins, 0);
+    const double dk = 2 * M_PI / boxlen;
+    for (long n2 = 1; n2 <= n2max; n2++) {
+        if (!hist[n2] || lut[n2] < 0) continue;
+        obs[lut[n2]] += hist[n2];
+        ksum[lut[n2]] += (double)hist[n2] * (sqrt((double)n2) * dk);
+    }
+    double* dks = nullptr;
+    long long* dob = nullptr;
+    MITOS_HIP_CHECK(hipMalloc(&dks, sizeof(double) * bins));
+    MITOS_HIP_CHECK(hipMalloc(&dob, sizeof(long long) * bins));
+    MITOS_HIP_CHECK(hipMemcpy(dks, ksum.data(), sizeof(double) * bins, hipMemcpyHostToDevice));
+    MITOS_HIP_CHECK(hipMemcpy(dob, obs.data(), sizeof(long long) * bins, hipMemcpyHostToDevice));
+    PkGeometry g{dks, dob};
+    cache[key] = g;
+    *out = g;
+    return MITOS_OK;
+}
+
 // src/calc_powerspec.c:84-89: averages, empty bins give 0/0 = NaN
 void pk_finish(const double* sums, const long long* obs, int bins, double boxlen, double* k_out,
                double* p_out) {

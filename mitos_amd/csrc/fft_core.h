@@ -124,13 +124,15 @@ template <typename T, int L, int C, bool COLFAST>
 struct LdsMap {
     // COLFAST: lines are the fast index (strided passes, lanes walk columns)
     //   idx = p*(C+PADC) + c
-    // else: positions are the fast index (z passes)
-    //   idx = c*(L+PADL) + p
+    // else: positions are the fast index (z passes), one pad word after every
+    // 32 positions so the radix-R scatter of an exchange (stride R in p) hits
+    // distinct banks
+    //   idx = c*(L + L/32 + 1) + p + p/32
     static constexpr int PADC = (C >= 4) ? 1 : 0;
-    static constexpr int PADL = 1;
-    static constexpr int SIZE = COLFAST ? L * (C + PADC) : C * (L + PADL);
+    static constexpr int ROW = L + L / 32 + 1;
+    static constexpr int SIZE = COLFAST ? L * (C + PADC) : C * ROW;
     static __device__ __forceinline__ int idx(int p, int c) {
-        return COLFAST ? p * (C + PADC) + c : c * (L + PADL) + p;
+        return COLFAST ? p * (C + PADC) + c : c * ROW + p + (p >> 5);
     }
 };
 

@@ -45,6 +45,18 @@ __device__ __forceinline__ double bm_normal(Xo& gen) {
     return sqrt(-2 * log(u)) * cos(2 * M_PI * v);
 }
 
+// The same Box-Muller pair in float arithmetic (FP32 pipeline): the uniforms
+// are rounded to float and log, cos(2 pi v) = cospi(2 v) and sqrt are float
+// functions; the draws consumed are those of bm_normal.
+__device__ __forceinline__ float bm_normal_f(Xo& gen) {
+    const uint64_t A = xo_next(gen);
+    const uint64_t B = xo_next(gen);
+    const float s = 5.42101086242752217e-20f;  // 2^-64
+    const float u = ((float)A + 0.5f) * s;
+    const float v = ((float)B + 0.5f) * s;
+    return sqrtf(-2.0f * logf(u)) * cospif(2.0f * v);
+}
+
 // src/fft.c:29-35, evaluated without contraction so k matches the host bits
 __host__ __device__ __forceinline__ void wavevec(int x, int y, int z, int N, double dk,
                                                  double& kx, double& ky, double& kz,
@@ -88,8 +100,14 @@ __global__ void __launch_bounds__(256)
                 wavevec(x, y, z, N, dk, kx, ky, kz, k);
                 cx<T> val = mk<T>(0, 0);
                 if (k > 0) {
-                    const double a = bm_normal<CR>(gen) * factor;
-                    const double b = bm_normal<CR>(gen) * factor;
+                    double a, b;
+                    if constexpr (sizeof(T) == 4) {
+                        a = (double)bm_normal_f(gen) * factor;
+                        b = (double)bm_normal_f(gen) * factor;
+                    } else {
+                        a = bm_normal<CR>(gen) * factor;
+                        b = bm_normal<CR>(gen) * factor;
+                    }
                     const double m = amp(n2xy + (long)z * z, kx, ky, kz, k);
                     val = mk<T>((T)(a * m), (T)(b * m));
                 }

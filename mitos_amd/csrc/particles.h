@@ -321,53 +321,90 @@ __device__ __forceinline__ void tsc_axis_near_d(M X, int& c, M w[3], M dw[3]) {
 }
 
 // T: grid and arithmetic type (float in the FP32 pipeline; double when the
-// FP64 pipeline runs in this form for the adjoint).
-template <typename T, int NT, int TX>
+// FP64 pipeline runs in this form for the adjoint).  Each thread moves PPT
+// particles of consecutive lattice rows at the same z, phase by phase, so
+// the three dependent gathers of one particle overlap with the others'.
+template <typename T, int NT, int TX, int PPT>
 __global__ void __launch_bounds__(NT)
     k_displace_lattice_s(const T* __restrict__ sx, const T* __restrict__ sy,
                          const T* __restrict__ sz, unsigned rs, int N, int nch, T Lf,
                          float spacing, double boxlen, float* __restrict__ X,
                          float* __restrict__ Y, float* __restrict__ Z) {
+    static_assert(TX % PPT == 0, "a thread's rows lie in one strip");
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
-    const unsigned rowi = bid / (unsigned)nch;
-    const int qz = (int)(bid - rowi * (unsigned)nch) * NT + threadIdx.x;
+    const unsigned rowg = bid / (unsigned)nch;
+    const int qz = (int)(bid - rowg * (unsigned)nch) * NT + threadIdx.x;
     if (qz >= N) return;
-    const unsigned xi = rowi % TX, r2 = rowi / TX;
-    const int qy = (int)(r2 % (unsigned)N);
-    const int qx = (int)(r2 / (unsigned)N) * TX + (int)xi;
     const float invN = 1.0f / N;
     const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
-    const unsigned yz = (unsigned)qy * rs + MITOS_PSI_HO + qz;
-    float x = qx * spacing, y = qy * spacing, z = qz * spacing;  // src/particle.c:107-109
-    // x move: the stencilled x displacement at the site
-    x = (float)((T)x - sx[(unsigned)qx * plane + yz]);
-    // y move: TSC along x at (x', q_y, q_z)
-    int cx;
-    T wx[3];
-    tsc_axis_near((T)x * Nf / Lf, cx, wx);
-    unsigned ox[3];
-    tap_rows(wrapf(cx, N, invN), N, plane, ox);
-    y = (float)((T)y - fma(wx[2], sy[ox[2] + yz], fma(wx[1], sy[ox[1] + yz], wx[0] * sy[ox[0] + yz])));
-    // z move: TSC along x and y at (x', y', q_z)
-    int cy;
-    T wy[3];
-    tsc_axis_near((T)y * Nf / Lf, cy, wy);
-    unsigned oy[3];
-    tap_rows(wrapf(cy, N, invN), N, rs, oy);
     const unsigned zo = MITOS_PSI_HO + qz;
-    T d = 0;
+    int qx[PPT], qy[PPT];
+    unsigned yz[PPT];
+    float x[PPT], y[PPT], z[PPT];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const unsigned o = ox[a] + zo;
-        const T r = fma(wy[2], sz[o + oy[2]], fma(wy[1], sz[o + oy[1]], wy[0] * sz[o + oy[0]]));
-        d = fma(wx[a], r, d);
+    for (int j = 0; j < PPT; j++) {
+        const unsigned rowi = rowg * PPT + j;
+        const unsigned xi = rowi % TX, r2 = rowi / TX;
+        qy[j] = (int)(r2 % (unsigned)N);
+        qx[j] = (int)(r2 / (unsigned)N) * TX + (int)xi;
+        yz[j] = (unsigned)qy[j] * rs + zo;
+        // src/particle.c:107-109
+        x[j] = qx[j] * spacing;
+        y[j] = qy[j] * spacing;
+        z[j] = qz * spacing;
     }
-    z = (float)((T)z - d);
-    const long long p = ((long long)qx * N + qy) * N + qz;
-    X[p] = (float)fwrap_fast(x, boxlen);
-    Y[p] = (float)fwrap_fast(y, boxlen);
-    Z[p] = (float)fwrap_fast(z, boxlen);
+    // x move: the stencilled x displacement at the site
+    T sxv[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) sxv[j] = sx[(unsigned)qx[j] * plane + yz[j]];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) x[j] = (float)((T)x[j] - sxv[j]);
+    // y move: TSC along x at (x', q_y, q_z)
+    T wx[PPT][3];
+    unsigned ox[PPT][3];
+    T syv[PPT][3];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        int cx;
+        tsc_axis_near((T)x[j] * Nf / Lf, cx, wx[j]);
+        tap_rows(wrapf(cx, N, invN), N, plane, ox[j]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) syv[j][a] = sy[ox[j][a] + yz[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++)
+        y[j] = (float)((T)y[j] -
+                       fma(wx[j][2], syv[j][2], fma(wx[j][1], syv[j][1], wx[j][0] * syv[j][0])));
+    // z move: TSC along x and y at (x', y', q_z)
+    T wy[PPT][3];
+    T szv[PPT][9];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        int cy;
+        tsc_axis_near((T)y[j] * Nf / Lf, cy, wy[j]);
+        unsigned oy[3];
+        tap_rows(wrapf(cy, N, invN), N, rs, oy);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) szv[j][a * 3 + b] = sz[ox[j][a] + oy[b] + zo];
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        T d = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T r = fma(wy[j][2], szv[j][a * 3 + 2],
+                            fma(wy[j][1], szv[j][a * 3 + 1], wy[j][0] * szv[j][a * 3]));
+            d = fma(wx[j][a], r, d);
+        }
+        z[j] = (float)((T)z[j] - d);
+        const long long p = ((long long)qx[j] * N + qy[j]) * N + qz;
+        X[p] = (float)fwrap_fast(x[j], boxlen);
+        Y[p] = (float)fwrap_fast(y[j], boxlen);
+        Z[p] = (float)fwrap_fast(z[j], boxlen);
+    }
 }
 
 // analyse/src/mitos_read.c:285-320: base cell and the eight CIC weights of a
@@ -601,6 +638,9 @@ __global__ void __launch_bounds__(NT)
                   const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid) {
     constexpr int P = PX * PY * PZ, K = P / NT, R = RX * RY * RZ;
     static_assert(P % NT == 0, "tile must be a multiple of the work-group");
+    // the flush wraps window cells with one (x, y) or two (z) subtractions of
+    // N, so the window fits in N = 32 (the smallest grid this kernel runs on)
+    static_assert(RX <= 32 && RY <= 32 && RZ <= 64, "window too large for N >= 32");
     __shared__ unsigned long long win[R];
     __shared__ int red[3][NT / 64];
     const int ntz = N / PZ, nty = N / PY;
@@ -675,12 +715,18 @@ __global__ void __launch_bounds__(NT)
         }
     }
     __syncthreads();
+    // window origin wrapped once; a window side is shorter than 2N
+    const int gx0 = wrapi(ox, N), gy0 = wrapi(oy, N), gz0 = wrapi(oz, N);
     for (int c = threadIdx.x; c < R; c += NT) {
         const unsigned long long v = win[c];
         if (v) {
             const int cz = c % RZ, cy = (c / RZ) % RY, cx = c / (RZ * RY);
-            const long gx = wrapi(ox + cx, N), gy = wrapi(oy + cy, N), gz = wrapi(oz + cz, N);
-            atomicAdd(grid + (gx * N + gy) * N + gz, v);
+            int gx = gx0 + cx, gy = gy0 + cy, gz = gz0 + cz;
+            gx = gx >= N ? gx - N : gx;
+            gy = gy >= N ? gy - N : gy;
+            gz = gz >= N ? gz - N : gz;
+            gz = gz >= N ? gz - N : gz;
+            atomicAdd(grid + ((long)gx * N + gy) * N + gz, v);
         }
     }
 }

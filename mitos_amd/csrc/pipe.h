@@ -376,6 +376,52 @@ struct Pipe {
         return MITOS_OK;
     }
 
+    // Mean time of a plain 3D r2c (real rows of N + MITOS_PSI_PAD -> K) and c2r
+    // (K -> real) over reps runs each, with the tuned schedules; uses the
+    // pipeline's grids as scratch.
+    int fft_timing(hipStream_t s, int reps, float* r2c_ms, float* c2r_ms) {
+        hipEvent_t e0, e1, e2;
+        MITOS_HIP_CHECK(hipEventCreate(&e0));
+        MITOS_HIP_CHECK(hipEventCreate(&e1));
+        MITOS_HIP_CHECK(hipEventCreate(&e2));
+        T* real = psi(0);
+        const long rs = N + MITOS_PSI_PAD;
+        const int rc = with_size(N, [&]<int NN>() -> int {
+            RealRowLoader<T> ld{real, rs, 0};
+            auto r2c = [&]() -> int {
+                MITOS_TRY((zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, K, hs, 1, 0, OpNone(),
+                                                  s)));
+                MITOS_TRY((strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, K, hs, 1, 0,
+                                                            OpNone(), s)));
+                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], K, K, hs, 1, 0,
+                                                        OpNone(), s);
+            };
+            auto c2r = [&]() -> int {
+                MITOS_TRY((strided_pass<T, NN, 0, 1>(K, a(0), hs, 1, 0, OpNone(), s)));
+                MITOS_TRY((strided_pass<T, NN, 1, 1>(a(0), a(0), hs, 1, 0, OpNone(), s)));
+                return zc2r_pass_tuned<T, NN, 0>(sched.v[TUNE_Z_C2R], a(0), real, hs, rs, 1, 0, 0,
+                                                 (T)1, OpNone(), s);
+            };
+            MITOS_TRY(r2c());
+            MITOS_TRY(c2r());
+            MITOS_HIP_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) MITOS_TRY(r2c());
+            MITOS_HIP_CHECK(hipEventRecord(e1, s));
+            for (int r = 0; r < reps; r++) MITOS_TRY(c2r());
+            MITOS_HIP_CHECK(hipEventRecord(e2, s));
+            MITOS_HIP_CHECK(hipEventSynchronize(e2));
+            MITOS_HIP_CHECK(hipEventElapsedTime(r2c_ms, e0, e1));
+            MITOS_HIP_CHECK(hipEventElapsedTime(c2r_ms, e1, e2));
+            *r2c_ms /= reps;
+            *c2r_ms /= reps;
+            return MITOS_OK;
+        });
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipEventDestroy(e2);
+        return rc;
+    }
+
     cx<T>* a(int i) { return A + (size_t)i * csize; }
     cx<T>* b(int i) { return B + (size_t)i * csize; }
     T* psi(int i) { return reinterpret_cast<T*>(b(3 + i)); }
@@ -611,9 +657,11 @@ struct Pipe {
                     const unsigned rs = (unsigned)(NN + MITOS_PSI_PAD);
                     if (STENCIL) {
                         constexpr int TX = NN % 16 == 0 ? 16 : 8;
-                        hipLaunchKernelGGL((k_displace_lattice_s<T, NT, TX>), grid, dim3(NT), 0,
-                                           s, psi(0), psi(1), psi(2), rs, NN, nch, (T)L, spacing,
-                                           L, X, Y, Z);
+                        constexpr int PPT = 4;
+                        const dim3 gridp((unsigned)((long)NN * NN / PPT * nch));
+                        hipLaunchKernelGGL((k_displace_lattice_s<T, NT, TX, PPT>), gridp,
+                                           dim3(NT), 0, s, psi(0), psi(1), psi(2), rs, NN, nch,
+                                           (T)L, spacing, L, X, Y, Z);
                     } else {
                         hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,
                                            psi(0), psi(1), psi(2), rs, NN, nch, L, spacing,

@@ -157,6 +157,14 @@ extern "C" int mitos_ic_schedule(mitos_ic_context* c, int kind, float* ms) {
     return sc.v[kind];
 }
 
+extern "C" int mitos_ic_fft_timing(mitos_ic_context* c, int reps, float* r2c_ms,
+                                   float* c2r_ms) {
+    if (!c || reps < 1 || !r2c_ms || !c2r_ms) return set_last_error(MITOS_ERR_ARG);
+    const int rc = c->pf ? c->pf->fft_timing(c->stream, reps, r2c_ms, c2r_ms)
+                         : c->pd->fft_timing(c->stream, reps, r2c_ms, c2r_ms);
+    return set_last_error(rc);
+}
+
 extern "C" int mitos_set_fft_schedule(int N, int precision, const int* variants) {
     if (!variants || !size_supported(N) || (precision != MITOS_FP32 && precision != MITOS_FP64))
         return set_last_error(MITOS_ERR_ARG);

@@ -51,28 +51,33 @@ __device__ __forceinline__ int pk_bin_of(long n2, const int* first, int bins, fl
     return b;
 }
 
-// Partials layout: part[(blk*bins + b)*4 + {sumP, compP, sumK, compK}], cnt[blk*bins + b]
+// Partials layout: part[(blk*bins + b)*2 + {sum, compensation}] of the power.
+// Mode counts and k sums depend only on (N, boxlen, bins); they come from
+// pk_geometry.
+//
+// A wavefront takes 64 consecutive z' of one row.  Along a row n^2 grows with
+// z', so equal bins form contiguous lane runs: a segmented inclusive scan
+// (fixed order) leaves each run's total in its last lane, which adds it to
+// the wavefront's compensated accumulator of that bin in LDS (different runs,
+// different bins, no conflicts).
 template <typename T, typename XF>
 __global__ void __launch_bounds__(256)
     k_pk_partial(const cx<T>* __restrict__ a, const cx<T>* __restrict__ b, int N, long hs,
-                 const int* __restrict__ edges, float cinv, double dk, int bins, double* part,
-                 long long* cnt, XF xf) {
+                 const int* __restrict__ edges, float cinv, int bins, double* part, XF xf) {
 #pragma clang fp contract(off)
-    // [4 waves][bins][4] doubles, [4][bins] counts, [bins + 1] bin edges
+    // [4 waves][bins][2] doubles, [bins + 1] bin edges
     extern __shared__ double sm[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    double* acc = sm + (size_t)wv * bins * 4;
-    long long* cacc = reinterpret_cast<long long*>(sm + (size_t)4 * bins * 4) + (size_t)wv * bins;
-    int* first = reinterpret_cast<int*>(sm + (size_t)4 * bins * 4 + (size_t)4 * bins);
-    for (int i = lane; i < bins * 4; i += 64) acc[i] = 0;
-    for (int i = lane; i < bins; i += 64) cacc[i] = 0;
+    double* acc = sm + (size_t)wv * bins * 2;
+    int* first = reinterpret_cast<int*>(sm + (size_t)4 * bins * 2);
+    for (int i = lane; i < bins * 2; i += 64) acc[i] = 0;
     for (int i = threadIdx.x; i <= bins; i += blockDim.x) first[i] = edges[i];
     __syncthreads();
+    const bool same = (a == b);
     const int H = N / 2 + 1;
     const long nrows = (long)N * N;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-    // one wavefront per z-row at a time, 64 consecutive modes per step
     for (long row = (long)blockIdx.x * (blockDim.x >> 6) + wv; row < nrows; row += nwaves) {
         const int x = (int)(row / N), y = (int)(row % N);
         const int nx = freq_index(x, N), ny = freq_index(y, N);
@@ -82,81 +87,75 @@ __global__ void __launch_bounds__(256)
         const cx<T>* rb = b + row * hs;
         for (int z0 = 0; z0 < H; z0 += 64) {
             const int z = z0 + lane;
-            int mybin = -1;
-            double vP = 0, vK = 0;
-            const bool edge = (z == 0 || z == N / 2);
+            int bin = -1;
+            double v = 0;
             if (z < H) {
                 const long n2 = n2xy + (long)z * z;
                 if (n2 > 0) {
                     const double f = fr * xf.col(z);
-                    const cx<T> u = ra[z], v = rb[z];
-                    const double mult = edge ? 1.0 : 2.0;
-                    mybin = pk_bin_of(n2, first, bins, cinv);
-                    // (f u).(f v) with the factor applied to each operand
-                    vP = mult * (((double)u.re * f) * ((double)v.re * f) +
-                                 ((double)u.im * f) * ((double)v.im * f));
-                    vK = mult * (sqrt((double)n2) * dk);
+                    const cx<T> u = ra[z];
+                    const cx<T> w = same ? u : rb[z];
+                    const double mult = (z == 0 || z == N / 2) ? 1.0 : 2.0;
+                    bin = pk_bin_of(n2, first, bins, cinv);
+                    // (f u).(f w) with the factor applied to each operand
+                    v = mult * (((double)u.re * f) * ((double)w.re * f) +
+                                ((double)u.im * f) * ((double)w.im * f));
                 }
             }
-            unsigned long long act = __ballot(mybin >= 0);
-            while (act) {
-                const int lead = __ffsll((long long)act) - 1;
-                const int bl = __shfl(mybin, lead, 64);
-                const bool mine = (mybin == bl);
-                const unsigned long long m = __ballot(mine);
-                const unsigned long long me = __ballot(mine && edge);
-                const double sP = wave_sum(mine ? vP : 0.0);
-                const double sK = wave_sum(mine ? vK : 0.0);
-                if (lane == 0) {
-                    kadd(acc[bl * 4 + 0], acc[bl * 4 + 1], sP);
-                    kadd(acc[bl * 4 + 2], acc[bl * 4 + 3], sK);
-                    cacc[bl] += 2LL * __popcll(m) - __popcll(me);
-                }
-                act &= ~m;
+            // segmented inclusive scan over the lane runs of equal bin
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double vo = __shfl_up(v, o, 64);
+                const int bo = __shfl_up(bin, o, 64);
+                if (lane >= o && bo == bin) v += vo;
             }
+            const int bnext = __shfl_down(bin, 1, 64);
+            if (bin >= 0 && (lane == 63 || bnext != bin))
+                kadd(acc[bin * 2 + 0], acc[bin * 2 + 1], v);
         }
     }
     __syncthreads();
     for (int bn = threadIdx.x; bn < bins; bn += blockDim.x) {
-        double sP = 0, cP = 0, sK = 0, cK = 0;
-        long long c = 0;
+        double sP = 0, cP = 0;
         for (int w = 0; w < 4; w++) {
-            const double* aw = sm + (size_t)w * bins * 4 + bn * 4;
+            const double* aw = sm + (size_t)w * bins * 2 + bn * 2;
             kadd(sP, cP, aw[0]);
             kadd(sP, cP, aw[1]);
-            kadd(sK, cK, aw[2]);
-            kadd(sK, cK, aw[3]);
-            c += reinterpret_cast<long long*>(sm + (size_t)4 * bins * 4)[(size_t)w * bins + bn];
         }
-        double* o = part + ((size_t)blockIdx.x * bins + bn) * 4;
+        double* o = part + ((size_t)blockIdx.x * bins + bn) * 2;
         o[0] = sP;
         o[1] = cP;
-        o[2] = sK;
-        o[3] = cK;
-        cnt[(size_t)blockIdx.x * bins + bn] = c;
     }
 }
 
-// fold the partials of nblk work-groups, in order; out[b*3 + {P_sum, K_sum}], obs[b]
-static __global__ void k_pk_final(const double* part, const long long* cnt, int nblk, int bins,
-                           double* out, long long* obs) {
+// fold the partials of nblk work-groups, in order; out[b*2 + {P_sum, K_sum}],
+// obs[b] (the k sums and counts from the geometry tables)
+static __global__ void k_pk_final(const double* part, int nblk, int bins,
+                                  const double* __restrict__ ksum,
+                                  const long long* __restrict__ kobs, double* out,
+                                  long long* obs) {
 #pragma clang fp contract(off)
     const int bn = blockIdx.x * blockDim.x + threadIdx.x;
     if (bn >= bins) return;
-    double sP = 0, cP = 0, sK = 0, cK = 0;
-    long long c = 0;
+    double sP = 0, cP = 0;
     for (int g = 0; g < nblk; g++) {
-        const double* o = part + ((size_t)g * bins + bn) * 4;
+        const double* o = part + ((size_t)g * bins + bn) * 2;
         kadd(sP, cP, o[0]);
         kadd(sP, cP, o[1]);
-        kadd(sK, cK, o[2]);
-        kadd(sK, cK, o[3]);
-        c += cnt[(size_t)g * bins + bn];
     }
     out[bn * 2 + 0] = sP + cP;
-    out[bn * 2 + 1] = sK + cK;
-    obs[bn] = c;
+    out[bn * 2 + 1] = ksum[bn];
+    obs[bn] = kobs[bn];
 }
+
+// Host: per-bin mode counts (with the multiplicity of src/calc_powerspec.c:
+// 75-80) and sums of multiplicity * |k|, as device arrays, cached per
+// (device, N, boxlen, bins).  Made from an integer histogram over n^2.
+struct PkGeometry {
+    const double* ksum;
+    const long long* obs;
+};
+int pk_geometry(int N, double boxlen, int bins, PkGeometry* out);
 
 // Host: device table first[b], b in [0, bins], of the smallest n^2 in
 // [1, 3(N/2)^2] whose reference bin is >= b (first[bins] = 3(N/2)^2 + 1).
@@ -183,27 +182,26 @@ inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen,
     int err = MITOS_OK;
     const int* edges = pk_bin_edges(N, boxlen, bins, &err);
     if (!edges) return err;
+    PkGeometry geo;
+    MITOS_TRY(pk_geometry(N, boxlen, bins, &geo));
     const long nmodes = (long)N * N * (N / 2 + 1);
     const int nblk = pk_blocks(nmodes);
-    const size_t pbytes = sizeof(double) * 4 * bins * nblk;
-    const size_t cbytes = sizeof(long long) * bins * nblk;
+    const size_t pbytes = sizeof(double) * 2 * bins * nblk;
     char* ws = nullptr;
     if (!part_ws || !dev_sums) {
-        ws = (char*)workspace(2, pbytes + cbytes + sizeof(double) * 2 * bins +
-                                     sizeof(long long) * bins);
+        ws = (char*)workspace(2, pbytes + sizeof(double) * 2 * bins + sizeof(long long) * bins);
         if (!ws) return MITOS_ERR_ALLOC;
     }
+    (void)cnt_ws;
     double* part = part_ws ? part_ws : (double*)ws;
-    long long* cnt = part_ws ? cnt_ws : (long long*)(ws + pbytes);
-    double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes + cbytes);
-    long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + cbytes + sizeof(double) * 2 * bins);
-    const size_t shm =
-        sizeof(double) * 4 * 4 * bins + sizeof(long long) * 4 * bins + sizeof(int) * (bins + 1);
+    double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes);
+    long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + sizeof(double) * 2 * bins);
+    const size_t shm = sizeof(double) * 4 * 2 * bins + sizeof(int) * (bins + 1);
     hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, edges,
-                       pk_bin_cinv(N, bins), 2 * M_PI / boxlen, bins, part, cnt, xf);
+                       pk_bin_cinv(N, bins), bins, part, xf);
     MITOS_TRY(launch_status());
-    hipLaunchKernelGGL(k_pk_final, dim3((bins + 63) / 64), dim3(64), 0, s, part, cnt, nblk,
-                       bins, fs, fo);
+    hipLaunchKernelGGL(k_pk_final, dim3((bins + 63) / 64), dim3(64), 0, s, part, nblk, bins,
+                       geo.ksum, geo.obs, fs, fo);
     MITOS_TRY(launch_status());
     if (sums) {
         MITOS_HIP_CHECK(hipMemcpyAsync(sums, fs, sizeof(double) * 2 * bins,

@@ -256,6 +256,15 @@ class IC:
             raise ValueError(f"noise must be a contiguous CUDA {want} tensor of shape "
                              f"({N}, {N}, {N // 2 + 1})")
 
+    def fft_timing(self, reps: int = 5) -> tuple:
+        """(r2c ms, c2r ms) of a plain 3D transform of this size (overwrites the
+        context's grids)."""
+        a = ctypes.c_float()
+        b = ctypes.c_float()
+        check(lib().mitos_ic_fft_timing(self.h, reps, ctypes.byref(a), ctypes.byref(b)),
+              "mitos_ic_fft_timing")
+        return a.value, b.value
+
     def schedules(self) -> dict:
         """Autotuned FFT schedule per pass kind: (variant, [ms per variant])."""
         out = {}

@@ -441,3 +441,51 @@ def test_powerspec_white_noise_expectation(oracle):
     ok = obs > 500
     mean = P[ok] * L ** 3 / N ** 3
     assert np.all(np.abs(mean - 1) < 5 / np.sqrt(obs[ok]) + 0.02)
+
+
+# ---- the FP64 reference gradient (oracle/adjoint_ref.py) ------------------------------
+def test_adjoint_reference_vs_finite_differences(oracle):
+    import torch
+    from oracle import adjoint_ref as R
+    N, L, bins = 8, 64.0, 6
+    fw = R.Forward(N, L, bins, oracle.sigma8_amplitude(0.965, 0.21, 0.81))
+    rng = np.random.default_rng(1)
+    H = N // 2 + 1
+    noise = rng.standard_normal((N, N, H)) + 1j * rng.standard_normal((N, N, H))
+    with torch.no_grad():
+        P0 = fw.power(fw.density(torch.tensor(noise))).numpy()
+    tgt = P0 * (1 + 0.1 * rng.standard_normal(bins))
+    _, g = R.pk_gradient(fw, noise, tgt)
+
+    def loss(nz):
+        with torch.no_grad():
+            return float(R.pk_loss(fw.power(fw.density(torch.tensor(nz))), tgt))
+
+    for (x, y, z) in [(1, 2, 1), (3, 5, 2), (6, 1, 3), (5, 5, 0), (2, 2, 0), (0, 1, 4)]:
+        for d in (1e-6, 1e-6j):
+            p = noise.copy()
+            p[x, y, z] += d
+            lp = loss(p)
+            p[x, y, z] -= 2 * d
+            fd = (lp - loss(p)) / 2e-6
+            ga = g[x, y, z].real if d.real else g[x, y, z].imag
+            assert abs(fd - ga) <= 1e-6 * np.abs(g).max(), (x, y, z, d, fd, ga)
+
+
+def test_adjoint_reference_forward_matches_oracle_pipeline(oracle):
+    # the torch restatement of the forward (stencil form, float64 positions)
+    # agrees with the oracle's reference-order pipeline to the rounding of the
+    # oracle's float positions
+    import torch
+    from oracle import adjoint_ref as R
+    N, L, bins = 16, 64.0, 8
+    A = oracle.sigma8_amplitude(0.965, 0.21, 0.81)
+    fw = R.Forward(N, L, bins, A)
+    # the seed-42 noise: the oracle GRF with sqrt(L^3/2) = 1
+    noise = oracle.grf(N, 2.0 ** (1.0 / 3.0), 42, hermitian=False)
+    with torch.no_grad():
+        P = fw.power(fw.density(torch.tensor(noise))).numpy()
+    ref = oracle.ic_run(N, L, 42, bins=bins)
+    ok = ref["obs"] > 0
+    assert np.array_equal(fw.obs[ok], ref["obs"][ok])
+    assert np.max(np.abs(P[ok] / ref["P"][ok] - 1)) < 1e-4
