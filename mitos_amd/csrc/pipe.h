@@ -217,7 +217,9 @@ template <typename T, int N, typename RAD = typename LineCfgS<N / 2>::R>
 struct HessZK;
 template <typename T, int N, int... Rs>
 struct HessZK<T, N, Radices<Rs...>> {
-    static int go(const MultiPtrs<T, 6, 1>& P, long hs, double dk, T scale, hipStream_t s) {
+    // delta2 rows written as reals (row stride rs) for a separate z pass
+    static int go(const MultiPtrs<T, 6, 1>& P, long hs, double dk, T scale, T* real, long rs,
+                  hipStream_t s) {
         constexpr int L = N / 2;
         constexpr int E = LineCfgS<L>::E;
         constexpr int TPL = L / E;
@@ -226,8 +228,8 @@ struct HessZK<T, N, Radices<Rs...>> {
         const cx<T>* tw = twiddles<T>(L);
         const cx<T>* twN = twiddles<T>(N);
         if (!tw || !twN) return MITOS_ERR_ALLOC;
-        hipLaunchKernelGGL((k_hess_z<T, L, E, RW, Rs...>), dim3(nblk(nrows, RW)), dim3(TPL * RW),
-                           0, s, P, nrows, hs, dk, scale, tw, twN);
+        hipLaunchKernelGGL((k_hess_z<T, L, E, RW, Rs...>), dim3(nblk(nrows, RW)),
+                           dim3(TPL * RW), 0, s, P, nrows, hs, dk, scale, tw, twN, real, rs);
         return launch_status();
     }
 };
@@ -561,8 +563,16 @@ struct Pipe {
                 MultiPtrs<T, 6, 1> p4;
                 for (int i = 0; i < 6; i++) p4.in[i] = b(i);
                 p4.out[0] = D;
-                MITOS_TRY(timed(s, "hessian_z_delta2", 7 * G, [&] {
-                    return HessZK<T, NN>::go(p4, hs, dk, (T)(1.0 / boxvol), s);
+                // delta2(x) rows into a(1) as reals, then their r2c into D
+                T* d2 = reinterpret_cast<T*>(a(1));
+                const long rs2 = NN + MITOS_PSI_PAD;
+                MITOS_TRY(timed(s, "hessian_z_delta2", 6 * G + Rg, [&] {
+                    return HessZK<T, NN>::go(p4, hs, dk, (T)(1.0 / boxvol), d2, rs2, s);
+                }));
+                MITOS_TRY(timed(s, "delta2_z_r2c", Rg + G, [&] {
+                    RealRowLoader<T> ld{d2, rs2, 0};
+                    return zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, D, hs, 1, 0, OpNone(),
+                                                  s);
                 }));
                 // 5. y pass forward of delta2
                 MITOS_TRY(timed(s, "delta2_y_fwd", 2 * G, [&] {

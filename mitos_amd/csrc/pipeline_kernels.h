@@ -95,15 +95,18 @@ __global__ void __launch_bounds__(L / E * C)
 }
 
 
-// ---- z pass: six Hessian rows -> delta2 -> r2c row ----------------------------
+// ---- z pass: six Hessian rows -> delta2 row (reals) ---------------------------- ----------------------------
 // Inputs (rows of hs complex, (x, y, k_z) space): B[0] = Y(A0), B[1] = Y(i ky A0),
 // B[2] = Y(-ky^2 A0), B[3] = Y(A1), B[4] = Y(i ky A1), B[5] = Y(A2).
 // Hessian rows: xx = B5, yy = B2, zz = -kz^2 B0, xy = B4, yz = i kz B1,
 // xz = i kz B3.  delta2 follows src/2lpt.c:111-112 term by term.
+// The delta2 row is written as N reals (row stride rs); its r2c runs as a
+// separate z pass (a fused r2c here doubles the live registers).
 template <typename T, int L, int E, int RW, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
     k_hess_z(MultiPtrs<T, 6, 1> P, long nrows, long hs, double dk, T scale,
-             const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN) {
+             const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN, T* __restrict__ real,
+             long rs) {
     constexpr int TPL = L / E;
     typedef LdsMap<T, L, RW, false> M;
     __shared__ T lre[M::SIZE];
@@ -112,9 +115,11 @@ __global__ void __launch_bounds__(L / E * RW)
     const int rl = threadIdx.x / TPL;
     const long row = (long)blockIdx.x * RW + rl;
     const bool ok = row < nrows;
-    cx<T> xx[E], yy[E], s[E], cur[E];
+    // xx -> A; yy -> B = xx + yy, A = xx * yy; zz -> A = zz * B + A; then
+    // A -= h^2 for xy, yz, xz: two accumulator arrays besides the transform
+    cx<T> A[E], B[E], cur[E];
     // component order: xx, yy, zz, xy, yz, xz
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 6; q++) {
         const int src_q = (q == 0) ? 5 : (q == 1) ? 2 : (q == 2) ? 0 : (q == 3) ? 4 : (q == 4) ? 1 : 3;
         const int mul_q = (q == 2) ? 2 : (q >= 4) ? 1 : 0;
@@ -152,45 +157,31 @@ __global__ void __launch_bounds__(L / E * RW)
         for (int e = 0; e < E; e++) {
             // real values x[2m] = re, x[2m+1] = im
             const cx<T> h = mk<T>(cur[e].re * scale, cur[e].im * scale);
-            if (q == 0) xx[e] = h;
-            else if (q == 1) yy[e] = h;
-            else if (q == 2) {
-                // d_dzz*d_dyy + d_dzz*d_dxx + d_dyy*d_dxx
-                s[e].re = h.re * yy[e].re + h.re * xx[e].re + yy[e].re * xx[e].re;
-                s[e].im = h.im * yy[e].im + h.im * xx[e].im + yy[e].im * xx[e].im;
+            if (q == 0) {
+                A[e] = h;
+            } else if (q == 1) {
+                B[e] = mk<T>(A[e].re + h.re, A[e].im + h.im);
+                A[e] = mk<T>(A[e].re * h.re, A[e].im * h.im);
+            } else if (q == 2) {
+                // d_dzz*d_dyy + d_dzz*d_dxx + d_dyy*d_dxx = d_dzz*(d_dxx+d_dyy) + d_dxx*d_dyy
+                A[e].re = h.re * B[e].re + A[e].re;
+                A[e].im = h.im * B[e].im + A[e].im;
             } else {
                 // - d_dxy^2 - d_dyz^2 - d_dxz^2
-                s[e].re = s[e].re - h.re * h.re;
-                s[e].im = s[e].im - h.im * h.im;
+                A[e].re = A[e].re - h.re * h.re;
+                A[e].im = A[e].im - h.im * h.im;
             }
         }
     }
-    // r2c of delta2 (packed even/odd) -> k_z row
-    line_fft<T, L, E, RW, false, -1, Rs...>(s, lre, lim, tw, t, rl);
+    if (!ok) return;
+    T* dst = real + row * rs;
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int m = t + TPL * e;
-        lre[M::idx(m, rl)] = s[e].re;
-        lim[M::idx(m, rl)] = s[e].im;
-    }
-    __syncthreads();
-    if (!ok) return;
-    cx<T>* dst = P.out[0] + row * hs;
-    const T h = (T)0.5;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int k = t + TPL * e;
-        const int km = (k == 0) ? 0 : L - k;
-        cx<T> zk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
-        cx<T> zm = conjg(mk<T>(lre[M::idx(km, rl)], lim[M::idx(km, rl)]));
-        cx<T> ev = cscale(zk + zm, h);
-        cx<T> od = cscale(zk - zm, h);
-        cx<T> xk = ev + mul_i<-1>(cmul(twN[k], od));
-        st(dst + k, xk);
-    }
-    if (t == 0) {
-        cx<T> z0 = mk<T>(lre[M::idx(0, rl)], lim[M::idx(0, rl)]);
-        st(dst + L, mk<T>(z0.re - z0.im, (T)0));
+        typename vec2<T>::type p;
+        p.x = A[e].re;
+        p.y = A[e].im;
+        *reinterpret_cast<typename vec2<T>::type*>(dst + 2 * m) = p;
     }
 }
 
@@ -227,32 +218,41 @@ __global__ void __launch_bounds__(L / E * C)
     for (int e = 0; e < E; e++)
         v[e] = ok ? ld(D + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
     line_fft<T, L, E, C, true, -1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-    const double ky = freq_index(y, L) * cp.dk;
-    const double kz = (ok ? col : 0) * cp.dk;
-    const double wyz = cp.win ? cp.win[y] * cp.win[ok ? col : 0] : 1.0;
-    const double kyz2 = ky * ky + kz * kz;
+    // per-mode arithmetic in the grid's precision (double for FP64)
+    typedef T W;
+    const W ky = (W)(freq_index(y, L) * cp.dk);
+    const W kz = (W)((ok ? col : 0) * cp.dk);
+    const W wyz = cp.win ? (W)(cp.win[y] * cp.win[ok ? col : 0]) : (W)1;
+    const W kyz2 = ky * ky + kz * kz;
+    const W f1 = (W)cp.f1, f2sD = (W)cp.f2 * (W)cp.sD;
     cx<T> u[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int x = t + TPL * e;
-        const double kx = freq_index(x, L) * cp.dk;
-        const double k2 = kx * kx + kyz2;
+        const W kx = (W)(freq_index(x, L) * cp.dk);
+        const W k2 = kx * kx + kyz2;
         const cx<T> phi1 = ok ? ld(K + base + (long)x * pstride) : mk<T>(0, 0);
         // phi2 = inv_poisson(delta2(k)) (include/fft_kernels.h:96-99), DC x 1
-        const double ip = (k2 > 0) ? -1.0 / k2 : 1.0;
-        double pre = (double)phi1.re * cp.f1 + cp.f2 * ((double)v[e].re * cp.sD * ip);
-        double pim = (double)phi1.im * cp.f1 + cp.f2 * ((double)v[e].im * cp.sD * ip);
+        const W ip = (k2 > 0) ? (W)-1 / k2 : (W)1;
+        W pre, pim;
+        if constexpr (sizeof(T) == 8) {
+            pre = phi1.re * cp.f1 + cp.f2 * (v[e].re * cp.sD * ip);
+            pim = phi1.im * cp.f1 + cp.f2 * (v[e].im * cp.sD * ip);
+        } else {
+            pre = phi1.re * f1 + f2sD * (v[e].re * ip);
+            pim = phi1.im * f1 + f2sD * (v[e].im * ip);
+        }
         if (cp.win && k2 > 0) {
-            const double iw = cp.win[x] * wyz;  // 1/W_TSC(k), include/fft_kernels.h:198-219
+            const W iw = (W)cp.win[x] * wyz;  // 1/W_TSC(k), include/fft_kernels.h:198-219
             pre *= iw;
             pim *= iw;
         }
         if (cp.smooth) {
-            const double sz = cp.smooth[ok ? col : 0];
+            const W sz = (W)cp.smooth[ok ? col : 0];
             pre *= sz;
             pim *= sz;
             // x displacement: i kx S(kx) S(ky) times the value
-            const double kxs = kx * (cp.smooth[x] * cp.smooth[y]);
+            const W kxs = kx * (W)(cp.smooth[x] * cp.smooth[y]);
             u[e] = mk<T>((T)(-pim * kxs), (T)(pre * kxs));
         }
         v[e] = mk<T>((T)pre, (T)pim);
