@@ -292,8 +292,11 @@ int mitos_ic_create(const struct mitos_ic_config *cfg, mitos_ic_context **ctx);
 /* run one realization; cfg must keep N and precision of the context */
 int mitos_ic_run(mitos_ic_context *ctx, const struct mitos_ic_config *cfg,
                  struct mitos_ic_result *res);
-/* the same, issued on `stream` without a host synchronisation */
+/* the same, issued on `stream` (a hipStream_t; NULL: the context's stream)
+ * without a host synchronisation */
 int mitos_ic_enqueue(mitos_ic_context *ctx, const struct mitos_ic_config *cfg, void *stream);
+/* P(k) of the last enqueued realization; this call, and the copies below,
+ * first wait for the stream that realization was enqueued on */
 int mitos_ic_fetch_pk(mitos_ic_context *ctx, struct mitos_ic_result *res);
 /* copy the last run's particle positions (N^3 floats each) into caller
  * device arrays */

@@ -381,13 +381,14 @@ extern "C" int solve2LPT(struct distributed_grid* potential, struct distributed_
         rc = fft_apply_kernel_dg(&d, potential, derivatives[index_a[j]], NULL);
         if (rc == MITOS_OK) rc = fft_apply_kernel_dg(&d, &d, derivatives[index_b[j]], NULL);
         if (rc == MITOS_OK) rc = fft_c2r_dg(&d);
+        // on the library stream: the next derivative overwrites d
         if (rc == MITOS_OK &&
-            hipMemcpy(hess[j], d.box, sizeof(double) * padded, hipMemcpyDeviceToDevice) !=
-                hipSuccess)
+            hipMemcpyAsync(hess[j], d.box, sizeof(double) * padded, hipMemcpyDeviceToDevice,
+                           api_stream()) != hipSuccess)
             rc = MITOS_ERR_DEVICE;
     }
     if (rc == MITOS_OK &&
-        hipMemcpy(hdev, hess, sizeof(double*) * 6, hipMemcpyHostToDevice) != hipSuccess)
+        copy_sync(hdev, hess, sizeof(double*) * 6, hipMemcpyHostToDevice) != hipSuccess)
         rc = MITOS_ERR_DEVICE;
     if (rc == MITOS_OK) {
         hipLaunchKernelGGL(k_delta2, dim3(nblk(padded)), dim3(256), 0, api_stream(),

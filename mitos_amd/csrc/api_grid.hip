@@ -50,7 +50,7 @@ int noise_jumps(int N, const Xo& s0, NoiseJumps* out) {
         }
         uint64_t* d = nullptr;
         MITOS_HIP_CHECK(hipMalloc(&d, tab.size() * sizeof(uint64_t)));
-        MITOS_HIP_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(uint64_t),
+        MITOS_HIP_CHECK(copy_sync(d, tab.data(), tab.size() * sizeof(uint64_t),
                                   hipMemcpyHostToDevice));
         ent = JumpEntry{d, nbits};
         std::lock_guard<std::mutex> lock(mu);
@@ -151,10 +151,10 @@ extern "C" int mitos_grid_to_host(const struct distributed_grid* dg, fftw_comple
                                   double* box_host) {
     if (!dg) return set_last_error(MITOS_ERR_ARG);
     if (fbox_host && dg->fbox)
-        MITOS_HIP_CHECK(hipMemcpy(fbox_host, dg->fbox, sizeof(cx<double>) * dg->local_size,
+        MITOS_HIP_CHECK(copy_sync(fbox_host, dg->fbox, sizeof(cx<double>) * dg->local_size,
                                   hipMemcpyDeviceToHost));
     if (box_host && dg->box)
-        MITOS_HIP_CHECK(hipMemcpy(box_host, dg->box, sizeof(double) * 2 * dg->local_size,
+        MITOS_HIP_CHECK(copy_sync(box_host, dg->box, sizeof(double) * 2 * dg->local_size,
                                   hipMemcpyDeviceToHost));
     return MITOS_OK;
 }
@@ -163,10 +163,10 @@ extern "C" int mitos_grid_from_host(struct distributed_grid* dg, const fftw_comp
                                     const double* box_host) {
     if (!dg) return set_last_error(MITOS_ERR_ARG);
     if (fbox_host && dg->fbox)
-        MITOS_HIP_CHECK(hipMemcpy(dg->fbox, fbox_host, sizeof(cx<double>) * dg->local_size,
+        MITOS_HIP_CHECK(copy_sync(dg->fbox, fbox_host, sizeof(cx<double>) * dg->local_size,
                                   hipMemcpyHostToDevice));
     if (box_host && dg->box)
-        MITOS_HIP_CHECK(hipMemcpy(dg->box, box_host, sizeof(double) * 2 * dg->local_size,
+        MITOS_HIP_CHECK(copy_sync(dg->box, box_host, sizeof(double) * 2 * dg->local_size,
                                   hipMemcpyHostToDevice));
     return MITOS_OK;
 }

@@ -75,7 +75,7 @@ const int* pk_bin_edges(int N, double boxlen, int bins, int* err) {
     int* d = nullptr;
     *err = MITOS_ERR_ALLOC;
     if (hipMalloc(&d, sizeof(int) * first.size()) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) !=
+    if (copy_sync(d, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice) !=
         hipSuccess) {
         hipFree(d);
         return nullptr;
@@ -136,8 +136,9 @@ int pk_geometry(int N, double boxlen, int bins, PkGeometry* out) {
     long long* dob = nullptr;
     MITOS_HIP_CHECK(hipMalloc(&dks, sizeof(double) * bins));
     MITOS_HIP_CHECK(hipMalloc(&dob, sizeof(long long) * bins));
-    MITOS_HIP_CHECK(hipMemcpy(dks, ksum.data(), sizeof(double) * bins, hipMemcpyHostToDevice));
-    MITOS_HIP_CHECK(hipMemcpy(dob, obs.data(), sizeof(long long) * bins, hipMemcpyHostToDevice));
+    MITOS_HIP_CHECK(copy_sync(dks, ksum.data(), sizeof(double) * bins, hipMemcpyHostToDevice));
+    MITOS_HIP_CHECK(
+        copy_sync(dob, obs.data(), sizeof(long long) * bins, hipMemcpyHostToDevice));
     PkGeometry g{dks, dob};
     cache[key] = g;
     *out = g;

@@ -125,7 +125,7 @@ static int ensemble_build(mitos_ensemble* e) {
         e->amplitude[i] = amplitude_cached(c.n_s, c.gamma, c.sigma8);
         const std::vector<double> h = phi1_host(c.N, c.boxlen, e->amplitude[i], c.n_s, c.gamma);
         MITOS_HIP_CHECK(hipMalloc(&e->amp[i], sizeof(double) * h.size()));
-        MITOS_HIP_CHECK(hipMemcpy(e->amp[i], h.data(), sizeof(double) * h.size(),
+        MITOS_HIP_CHECK(copy_sync(e->amp[i], h.data(), sizeof(double) * h.size(),
                                   hipMemcpyHostToDevice));
     }
     MITOS_HIP_CHECK(hipMalloc(&e->pk, sizeof(double) * 2 * e->bmax * n));

@@ -65,7 +65,7 @@ const cx<T>* twiddles(int L) {
         delete[] h;
         return nullptr;
     }
-    if (hipMemcpy(d, h, sizeof(cx<T>) * L, hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_sync(d, h, sizeof(cx<T>) * L, hipMemcpyHostToDevice) != hipSuccess) {
         delete[] h;
         hipFree(d);
         return nullptr;

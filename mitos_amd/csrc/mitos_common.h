@@ -38,6 +38,16 @@
 
 namespace mitos {
 
+// Copy and wait until the bytes are in place.  The library's kernels run on
+// non-blocking streams, which are not ordered against the null stream of a
+// plain hipMemcpy, and a hipMemcpy from pageable host memory or between
+// device buffers may return before the copy has completed.
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
+}
+
 template <typename T>
 struct cx {
     T re, im;

@@ -3,6 +3,10 @@
 
 #include "mitos_common.h"
 
+#include <map>
+#include <mutex>
+#include <type_traits>
+
 namespace mitos {
 
 // fixed-point unit of the CIC deposit: weights are summed as integers of
@@ -15,11 +19,15 @@ __host__ __device__ __forceinline__ double fwrap_d(double x, double L) {
     return fmod(fmod(x, L) + L, L);  // include/distributed_grid.h:27
 }
 
-// fwrap for |x| < L without fmod: every step of fmod(fmod(x,L)+L, L) is an
-// exact subtraction there, so the result has the same bits as fwrap_d.
+// fwrap_d out of line: the far-position fallback below stays out of the
+// unrolled particle loops
+__host__ __device__ __noinline__ inline double fwrap_d_far(double x, double L) { return fwrap_d(x, L); }
+
+// fwrap for -L < x < 2L without fmod: every step of fmod(fmod(x,L)+L, L) is
+// an exact subtraction there, so the result has the same bits as fwrap_d.
 __host__ __device__ __forceinline__ double fwrap_fast(double x, double L) {
 #pragma clang fp contract(off)
-    if (!(x > -L && x < 2 * L)) return fwrap_d(x, L);
+    if (!(x > -L && x < 2 * L)) return fwrap_d_far(x, L);
     const double m = (x >= L) ? x - L : x;  // fmod(x, L)
     const double a = m + L;
     double r = (a >= L) ? a - L : a;         // fmod(a, L), a in [0, 2L]
@@ -440,14 +448,26 @@ __device__ __forceinline__ void cic_weights(double dx, double dy, double dz,
     w[7] = (unsigned long long)(dx * dy * dz * S + 0.5);
 }
 
+__device__ __noinline__ inline float fwrap_float_far(float x, double L) {
+    return (float)fwrap_fast(x, L);
+}
+
+// Float periodic wrap of a float position: x itself on [0, Lf), Lf = (float)L
+// (positions written by the displacement kernels lie there), the double wrap
+// otherwise.
+__device__ __forceinline__ float fwrap_float(float x, float Lf, double L) {
+    return (x >= 0.0f && x < Lf) ? x : fwrap_float_far(x, L);
+}
+
 // Float variant of cic_cell/cic_weights (FP32 pipeline): the offset in the
 // cell is formed as X - iX * cell with one rounding, so it keeps the float
 // position's accuracy; weights in units of 2^-31 (a weight of 1 still fits).
 __device__ __forceinline__ void cic_cell_f(float X, float Y, float Z, int N, float cell,
                                            float cinv, double boxlen, int& iX, int& iY, int& iZ,
                                            float& dx, float& dy, float& dz) {
-    const float P[3] = {(float)fwrap_fast(X, boxlen), (float)fwrap_fast(Y, boxlen),
-                        (float)fwrap_fast(Z, boxlen)};
+    const float Lf = (float)boxlen;
+    const float P[3] = {fwrap_float(X, Lf, boxlen), fwrap_float(Y, Lf, boxlen),
+                        fwrap_float(Z, Lf, boxlen)};
     int I[3];
     float D[3];
 #pragma unroll
@@ -623,116 +643,313 @@ __global__ void __launch_bounds__(256)
     cic_global(grid, N, iX, iY, iZ, q);
 }
 
+// eight global atomic adds at base cell (iX, iY, iZ), each index in
+// [-N, 2N); 32-bit cell indices (N^3 < 2^32)
+__device__ __forceinline__ void cic_global_near(unsigned long long* grid, int N, int iX, int iY,
+                                                int iZ, const unsigned long long w[8]) {
+    const unsigned n = (unsigned)N;
+    const unsigned x0 = (unsigned)wrap1(iX, N), x1 = (unsigned)wrap1((int)x0 + 1, N);
+    const unsigned y0 = (unsigned)wrap1(iY, N), y1 = (unsigned)wrap1((int)y0 + 1, N);
+    const unsigned z0 = (unsigned)wrap1(iZ, N), z1 = (unsigned)wrap1((int)z0 + 1, N);
+    const unsigned r00 = (x0 * n + y0) * n, r10 = (x1 * n + y0) * n;
+    const unsigned r01 = (x0 * n + y1) * n, r11 = (x1 * n + y1) * n;
+    atomicAdd(grid + (r00 + z0), w[0]);
+    atomicAdd(grid + (r10 + z0), w[1]);
+    atomicAdd(grid + (r01 + z0), w[2]);
+    atomicAdd(grid + (r00 + z1), w[3]);
+    atomicAdd(grid + (r11 + z0), w[4]);
+    atomicAdd(grid + (r10 + z1), w[5]);
+    atomicAdd(grid + (r01 + z1), w[6]);
+    atomicAdd(grid + (r11 + z1), w[7]);
+}
+
+// Cell of a lattice particle at site (qx, qy, qz) with position (x, y, z)
+// for the tiled deposit: base cell q + r with r the periodic cell offset in
+// [-N/2, N/2) (so the base index lies in [-N/2, 3N/2)), and the in-cell
+// offsets.
+template <typename W>
+__device__ __forceinline__ void cic_lattice_cell(float x, float y, float z, int N, double boxlen,
+                                                 float cell, float cinv, int qx, int qy, int qz,
+                                                 int& rx, int& ry, int& rz, W& fx, W& fy,
+                                                 W& fz) {
+    int iX, iY, iZ;
+    if constexpr (sizeof(W) == 8)
+        cic_cell(x, y, z, N, boxlen, iX, iY, iZ, fx, fy, fz);
+    else
+        cic_cell_f(x, y, z, N, cell, cinv, boxlen, iX, iY, iZ, fx, fy, fz);
+    rx = iX - qx;
+    ry = iY - qy;
+    rz = iZ - qz;
+    rx = rx < -N / 2 ? rx + N : (rx >= N / 2 ? rx - N : rx);
+    ry = ry < -N / 2 ? ry + N : (ry >= N / 2 ? ry - N : ry);
+    rz = rz < -N / 2 ? rz + N : (rz >= N / 2 ? rz - N : rz);
+}
+
+// LDS window word: 32 bits for float weights (2^-31 units: a carry out of a
+// cell is added to the grid as 2^32 at once), 64 bits for double weights
+template <typename W>
+struct CicWin {
+    typedef typename std::conditional<sizeof(W) == 4, unsigned, unsigned long long>::type type;
+};
+
+// grid index of window cell (ax, ay, az) at window origin (gx0, gy0, gz0)
+// (wrapped); the window is shorter than N on x and y and than 2N on z
+__device__ __forceinline__ unsigned cic_win_cell(int gx0, int gy0, int gz0, int ax, int ay,
+                                                 int az, int N) {
+    int gx = gx0 + ax, gy = gy0 + ay, gz = gz0 + az;
+    gx = gx >= N ? gx - N : gx;
+    gy = gy >= N ? gy - N : gy;
+    gz = gz >= N ? gz - N : gz;
+    gz = gz >= N ? gz - N : gz;
+    return ((unsigned)gx * (unsigned)N + (unsigned)gy) * (unsigned)N + (unsigned)gz;
+}
+
+// carries of the 32-bit window: corner c of the cloud at window cell
+// (ax, ay, az) wrapped past 2^32
+__device__ __noinline__ inline void cic_win_carry(unsigned long long* grid, unsigned mask,
+                                                  int gx0, int gy0, int gz0, int ax, int ay,
+                                                  int az, int N) {
+    // corner order of cic_weights: 000, 100, 010, 001, 110, 101, 011, 111
+    const int cx[8] = {0, 1, 0, 0, 1, 1, 0, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1},
+              cz[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+    for (int c = 0; c < 8; c++)
+        if (mask & (1u << c))
+            atomicAdd(grid + cic_win_cell(gx0, gy0, gz0, ax + cx[c], ay + cy[c], az + cz[c], N),
+                      1ull << 32);
+}
+
 // CIC deposit (analyse/src/mitos_read.c:281-334) of particles stored in
-// lattice order (X[(qx*N + qy)*N + qz] belongs to lattice site q).  A
-// work-group owns a PX*PY*PZ tile of lattice sites and deposits them into an
-// LDS window of RX*RY*RZ cells placed at the tile's mean displacement.  The
-// window sums integers, so the order of the LDS and global atomics does not
-// change the result; particles whose cloud leaves the window go straight to
-// the grid, and the window's non-zero cells are added to the grid at the end.
+// lattice order (X[(qx*N + qy)*N + qz] belongs to lattice site q).  Each
+// work-group loops over PX*PY*PZ tiles of lattice sites (tile = blockIdx.x +
+// k*gridDim.x, a fixed count per group, so every group exits) and deposits a
+// tile into an LDS window of RX*RY*RZ cells placed at the tile's mean
+// displacement; the next tile's positions are loaded while the window is
+// flushed.  Window and grid sum integers, so the order of the LDS and global
+// atomics does not change the result.  A particle whose cloud leaves the
+// window is queued in LDS (wave-aggregated appends) and deposited with global
+// atomics after the flush; the flush adds every non-zero window cell to the
+// grid with one atomic and clears it.
 // W = double: the reference's double weights in units of 2^-40 (FP64
-// pipeline); W = float: float weights in units of 2^-31 (FP32 pipeline).
+// pipeline), 64-bit window; W = float: float weights in units of 2^-31 (FP32
+// pipeline), 32-bit window with carries.
 template <typename W, int PX, int PY, int PZ, int RX, int RY, int RZ, int NT>
 __global__ void __launch_bounds__(NT)
     k_cic_lattice(const float* __restrict__ X, const float* __restrict__ Y,
-                  const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid) {
-    constexpr int P = PX * PY * PZ, K = P / NT, R = RX * RY * RZ;
-    static_assert(P % NT == 0, "tile must be a multiple of the work-group");
+                  const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid,
+                  int ntiles) {
+    typedef typename CicWin<W>::type WT;
+    constexpr int P = PX * PY * PZ, K = P / NT, R = RX * RY * RZ, QCAP = 512;
+    static_assert(P % NT == 0 && NT % 64 == 0, "tile must be a multiple of the work-group");
+    static_assert(P <= 65536, "queue entries are 16-bit tile slots");
     // the flush wraps window cells with one (x, y) or two (z) subtractions of
     // N, so the window fits in N = 32 (the smallest grid this kernel runs on)
     static_assert(RX <= 32 && RY <= 32 && RZ <= 64, "window too large for N >= 32");
-    __shared__ unsigned long long win[R];
+    __shared__ WT win[R];
     __shared__ int red[3][NT / 64];
+    __shared__ unsigned short queue[QCAP];
+    __shared__ int qn;
     const int ntz = N / PZ, nty = N / PY;
-    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
-    const int tz = bid % ntz, ty = (bid / ntz) % nty, tx = bid / (ntz * nty);
     const float cell = (float)(boxlen / N), cinv = (float)(N / boxlen);
-    int bx[K], by[K], bz[K];
-    W fx[K], fy[K], fz[K];
-    int sx = 0, sy = 0, sz = 0;
-#pragma unroll
-    for (int i = 0; i < K; i++) {
-        const int p = threadIdx.x + NT * i;
-        const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
-        const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
-        const long long idx = ((long long)qx * N + qy) * N + qz;
-        int iX, iY, iZ;
-        if constexpr (sizeof(W) == 8)
-            cic_cell(X[idx], Y[idx], Z[idx], N, boxlen, iX, iY, iZ, fx[i], fy[i], fz[i]);
-        else
-            cic_cell_f(X[idx], Y[idx], Z[idx], N, cell, cinv, boxlen, iX, iY, iZ, fx[i], fy[i],
-                       fz[i]);
-        int rx = iX - qx, ry = iY - qy, rz = iZ - qz;
-        rx = rx < -N / 2 ? rx + N : (rx >= N / 2 ? rx - N : rx);
-        ry = ry < -N / 2 ? ry + N : (ry >= N / 2 ? ry - N : ry);
-        rz = rz < -N / 2 ? rz + N : (rz >= N / 2 ? rz - N : rz);
-        bx[i] = qx + rx;
-        by[i] = qy + ry;
-        bz[i] = qz + rz;
-        sx += rx;
-        sy += ry;
-        sz += rz;
-    }
+    const int lane = threadIdx.x & 63;
     for (int c = threadIdx.x; c < R; c += NT) win[c] = 0;
+    if (threadIdx.x == 0) qn = 0;
+    // positions of the group's first tile
+    float px[K], py[K], pz[K];
+    auto load = [&](int tile) {
+        const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        sx += __shfl_xor(sx, o, 64);
-        sy += __shfl_xor(sy, o, 64);
-        sz += __shfl_xor(sz, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = sx;
-        red[1][threadIdx.x >> 6] = sy;
-        red[2][threadIdx.x >> 6] = sz;
-    }
-    __syncthreads();
-    int mx = 0, my = 0, mz = 0;
-    for (int w = 0; w < NT / 64; w++) {
-        mx += red[0][w];
-        my += red[1][w];
-        mz += red[2][w];
-    }
-    const int ox = tx * PX + (int)rintf((float)mx / P) - (RX - PX - 1) / 2;
-    const int oy = ty * PY + (int)rintf((float)my / P) - (RY - PY - 1) / 2;
-    const int oz = tz * PZ + (int)rintf((float)mz / P) - (RZ - PZ - 1) / 2;
+        for (int i = 0; i < K; i++) {
+            const int p = threadIdx.x + NT * i;
+            const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+            const unsigned idx =
+                ((unsigned)(tx * PX + lx) * N + (unsigned)(ty * PY + ly)) * N + tz * PZ + lz;
+            px[i] = X[idx];
+            py[i] = Y[idx];
+            pz[i] = Z[idx];
+        }
+    };
+    if ((int)blockIdx.x < ntiles) load(blockIdx.x);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+        // mean cell offset of the tile (cells are recomputed for the deposit)
+        int sx = 0, sy = 0, sz = 0;
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-        unsigned long long w[8];
-        cic_weights(fx[i], fy[i], fz[i], w);
-        const int ax = bx[i] - ox, ay = by[i] - oy, az = bz[i] - oz;
-        if (ax >= 0 && ax + 1 < RX && ay >= 0 && ay + 1 < RY && az >= 0 && az + 1 < RZ) {
-            unsigned long long* b = win + (ax * RY + ay) * RZ + az;
-            atomicAdd(b, w[0]);
-            atomicAdd(b + RY * RZ, w[1]);
-            atomicAdd(b + RZ, w[2]);
-            atomicAdd(b + 1, w[3]);
-            atomicAdd(b + RY * RZ + RZ, w[4]);
-            atomicAdd(b + RY * RZ + 1, w[5]);
-            atomicAdd(b + RZ + 1, w[6]);
-            atomicAdd(b + RY * RZ + RZ + 1, w[7]);
-        } else {
-            cic_global(grid, N, bx[i], by[i], bz[i], w);
+        for (int i = 0; i < K; i++) {
+            const int p = threadIdx.x + NT * i;
+            const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+            int rx, ry, rz;
+            W dx, dy, dz;
+            cic_lattice_cell(px[i], py[i], pz[i], N, boxlen, cell, cinv, tx * PX + lx,
+                             ty * PY + ly, tz * PZ + lz, rx, ry, rz, dx, dy, dz);
+            sx += rx;
+            sy += ry;
+            sz += rz;
         }
-    }
-    __syncthreads();
-    // window origin wrapped once; a window side is shorter than 2N
-    const int gx0 = wrapi(ox, N), gy0 = wrapi(oy, N), gz0 = wrapi(oz, N);
-    for (int c = threadIdx.x; c < R; c += NT) {
-        const unsigned long long v = win[c];
-        if (v) {
-            const int cz = c % RZ, cy = (c / RZ) % RY, cx = c / (RZ * RY);
-            int gx = gx0 + cx, gy = gy0 + cy, gz = gz0 + cz;
-            gx = gx >= N ? gx - N : gx;
-            gy = gy >= N ? gy - N : gy;
-            gz = gz >= N ? gz - N : gz;
-            gz = gz >= N ? gz - N : gz;
-            atomicAdd(grid + ((long)gx * N + gy) * N + gz, v);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+            sz += __shfl_xor(sz, o, 64);
         }
+        if (lane == 0) {
+            red[0][threadIdx.x >> 6] = sx;
+            red[1][threadIdx.x >> 6] = sy;
+            red[2][threadIdx.x >> 6] = sz;
+        }
+        __syncthreads();
+        int mx = 0, my = 0, mz = 0;
+        for (int w = 0; w < NT / 64; w++) {
+            mx += red[0][w];
+            my += red[1][w];
+            mz += red[2][w];
+        }
+        const int ox = tx * PX + (int)rintf((float)mx / P) - (RX - PX - 1) / 2;
+        const int oy = ty * PY + (int)rintf((float)my / P) - (RY - PY - 1) / 2;
+        const int oz = tz * PZ + (int)rintf((float)mz / P) - (RZ - PZ - 1) / 2;
+        const int gx0 = wrapi(ox, N), gy0 = wrapi(oy, N), gz0 = wrapi(oz, N);
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const int p = threadIdx.x + NT * i;
+            const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+            const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
+            int rx, ry, rz;
+            W dx, dy, dz;
+            cic_lattice_cell(px[i], py[i], pz[i], N, boxlen, cell, cinv, qx, qy, qz, rx, ry, rz,
+                             dx, dy, dz);
+            unsigned long long w[8];
+            cic_weights(dx, dy, dz, w);
+            const int ax = qx + rx - ox, ay = qy + ry - oy, az = qz + rz - oz;
+            const bool in = ax >= 0 && ax + 1 < RX && ay >= 0 && ay + 1 < RY && az >= 0 &&
+                            az + 1 < RZ;
+            if (in) {
+                WT* b = win + (ax * RY + ay) * RZ + az;
+                constexpr int off[8] = {0, RY * RZ, RZ, 1, RY * RZ + RZ, RY * RZ + 1, RZ + 1,
+                                        RY * RZ + RZ + 1};
+                if constexpr (sizeof(WT) == 4) {
+                    unsigned carry = 0;
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        const unsigned old = atomicAdd(b + off[c], (unsigned)w[c]);
+                        carry |= (old + (unsigned)w[c] < old) ? (1u << c) : 0u;
+                    }
+                    if (carry) cic_win_carry(grid, carry, gx0, gy0, gz0, ax, ay, az, N);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 8; c++) atomicAdd(b + off[c], (WT)w[c]);
+                }
+            }
+            const unsigned long long m = __ballot(!in);
+            if (m) {
+                const int lead = __ffsll((unsigned long long)m) - 1;
+                int base = 0;
+                if (lane == lead) base = atomicAdd(&qn, __popcll(m));
+                base = __shfl(base, lead, 64);
+                if (!in) {
+                    const int slot = base + __popcll(m & ((1ull << lane) - 1));
+                    if (slot < QCAP)
+                        queue[slot] = (unsigned short)(threadIdx.x + NT * i);
+                    else
+                        cic_global_near(grid, N, qx + rx, qy + ry, qz + rz, w);
+                }
+            }
+        }
+        __syncthreads();
+        // the next tile's positions load while this window is flushed
+        if (tile + (int)gridDim.x < ntiles) load(tile + gridDim.x);
+        // window cells to the grid and cleared; (cx, cy, cz) of cell c
+        // advance with c += NT
+        constexpr int DZ = NT % RZ, DY = (NT / RZ) % RY, DX = NT / (RZ * RY);
+        int cz = threadIdx.x % RZ, cy = (threadIdx.x / RZ) % RY, cx = threadIdx.x / (RZ * RY);
+        for (int c = threadIdx.x; c < R; c += NT) {
+            const WT v = win[c];
+            if (v) {
+                win[c] = 0;
+                atomicAdd(grid + cic_win_cell(gx0, gy0, gz0, cx, cy, cz, N),
+                          (unsigned long long)v);
+            }
+            cz += DZ;
+            cy += DY;
+            cx += DX;
+            if (cz >= RZ) {
+                cz -= RZ;
+                cy++;
+            }
+            if (cy >= RY) {
+                cy -= RY;
+                cx++;
+            }
+        }
+        // queued particles: the same cell and weights, recomputed
+        const int nq = qn < QCAP ? qn : QCAP;
+        for (int j = threadIdx.x; j < nq; j += NT) {
+            const int p = queue[j];
+            const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+            const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
+            const unsigned idx = ((unsigned)qx * N + (unsigned)qy) * N + qz;
+            int rx, ry, rz;
+            W dx, dy, dz;
+            cic_lattice_cell(X[idx], Y[idx], Z[idx], N, boxlen, cell, cinv, qx, qy, qz, rx, ry,
+                             rz, dx, dy, dz);
+            unsigned long long w[8];
+            cic_weights(dx, dy, dz, w);
+            cic_global_near(grid, N, qx + rx, qy + ry, qz + rz, w);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) qn = 0;
     }
 }
 
-// tile of the lattice-ordered deposit kernel (N must be a multiple of 32)
-#define MITOS_CIC_TILE 8, 16, 32, 16, 24, 40, 512
+// tiles of the lattice-ordered deposit kernel, 6 cells of window margin on
+// each side (about 0.4% of the particles of a z = 0 amplitude field leave
+// the window), 512 threads; N must be a multiple of 32.
+// float: 16 x 16 x 32 sites, 28 x 28 x 44 cells of 32 bits (135 KB of LDS);
+// double: 8 x 16 x 16 sites, 20 x 28 x 28 cells of 64 bits (123 KB).
+template <typename W>
+struct CicTile {
+    static constexpr int PX = 16, PY = 16, PZ = 32, RX = 28, RY = 28, RZ = 44, NT = 512;
+};
+template <>
+struct CicTile<double> {
+    static constexpr int PX = 8, PY = 16, PZ = 16, RX = 20, RY = 28, RZ = 28, NT = 512;
+};
+
+// the tile kernel of window type W
+template <typename W, typename C = CicTile<W>>
+constexpr auto cic_lattice_kernel() {
+    return k_cic_lattice<W, C::PX, C::PY, C::PZ, C::RX, C::RY, C::RZ, C::NT>;
+}
+
+// Launch the lattice deposit: as many work-groups as are resident at once
+// (per device and weight type), at most one per tile.
+template <typename W>
+inline int cic_lattice_launch(const float* X, const float* Y, const float* Z, int N,
+                              double boxlen, unsigned long long* grid, hipStream_t s) {
+    typedef CicTile<W> C;
+    if (N % 32 != 0) return MITOS_ERR_UNSUPPORTED;
+    const int ntiles = (N / C::PX) * (N / C::PY) * (N / C::PZ);
+    static std::mutex mu;
+    static std::map<int, int> resident;
+    int dev = 0;
+    MITOS_HIP_CHECK(hipGetDevice(&dev));
+    int groups = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = resident.find(dev);
+        if (it == resident.end()) {
+            int cus = 0, per = 0;
+            MITOS_HIP_CHECK(
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            MITOS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, cic_lattice_kernel<W>(), C::NT, 0));
+            if (cus <= 0 || per <= 0) return MITOS_ERR_DEVICE;
+            it = resident.emplace(dev, cus * per).first;
+        }
+        groups = it->second < ntiles ? it->second : ntiles;
+    }
+    hipLaunchKernelGGL(cic_lattice_kernel<W>(), dim3(groups), dim3(C::NT), 0, s, X, Y, Z, N,
+                       boxlen, grid, ntiles);
+    return MITOS_OK;
+}
 
 // analyse/src/mitos_read.c:363-376 for equal masses: delta = sum_w * N^3/npart - 1
 template <typename T>

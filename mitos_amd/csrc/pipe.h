@@ -62,7 +62,7 @@ inline const double* window_table(int N, double boxlen, int order) {
     }
     double* d = nullptr;
     if (hipMalloc(&d, sizeof(double) * N) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_sync(d, h.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(d);
         return nullptr;
     }
@@ -93,7 +93,7 @@ inline const V* smooth_table(int N) {
     }
     V* d = nullptr;
     if (hipMalloc(&d, sizeof(V) * N) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(V) * N, hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_sync(d, h.data(), sizeof(V) * N, hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(d);
         return nullptr;
     }
@@ -148,7 +148,8 @@ inline const double* phi1_table(int N, double boxlen, double A, double n_s, doub
     const std::vector<double> h = phi1_host(N, boxlen, A, n_s, gamma);
     double* d = nullptr;
     if (hipMalloc(&d, sizeof(double) * h.size()) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_sync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
         hipFree(d);
         return nullptr;
     }
@@ -682,9 +683,7 @@ struct Pipe {
                 // reads 3 float coordinates, writes the grid
                 MITOS_TRY(timed(s, "cic_deposit", 12.0 * np + 8.0 * np, [&] {
                     if (NN % 32 == 0) {
-                        hipLaunchKernelGGL((k_cic_lattice<M, MITOS_CIC_TILE>),
-                                           dim3((NN / 8) * (NN / 16) * (NN / 32)), dim3(512), 0,
-                                           s, X, Y, Z, NN, L, cic());
+                        MITOS_TRY(cic_lattice_launch<M>(X, Y, Z, NN, L, cic(), s));
                         return launch_status();
                     }
                     hipLaunchKernelGGL(k_cic<M>, dim3(nblk(np)), dim3(256), 0, s, X, Y, Z, np, NN, L,

@@ -18,6 +18,7 @@ struct mitos_ic_context {
     int bins;
     double boxlen;
     hipStream_t stream;
+    hipStream_t last = nullptr;  // stream of the last enqueued realization
     bool own_stream;
     bool timing;
     Pipe<float>* pf;
@@ -80,6 +81,13 @@ extern "C" int mitos_ic_destroy(mitos_ic_context* c) {
     return MITOS_OK;
 }
 
+// a realization enqueued on a caller stream has finished before the
+// context's own stream reads the pipeline buffers
+static int wait_caller_stream(mitos_ic_context* c) {
+    if (c->last && c->last != c->stream) MITOS_HIP_CHECK(hipStreamSynchronize(c->last));
+    return MITOS_OK;
+}
+
 static int check_cfg(mitos_ic_context* c, const struct mitos_ic_config* cfg) {
     if (!c || !cfg) return MITOS_ERR_ARG;
     if (cfg->N != c->N || cfg->precision != c->precision) return MITOS_ERR_DIMS;
@@ -105,6 +113,7 @@ extern "C" int mitos_ic_enqueue(mitos_ic_context* c, const struct mitos_ic_confi
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     c->bins = cfg->bins;
     c->boxlen = cfg->boxlen;
+    c->last = s;
     int rc = c->pf ? c->pf->enqueue(*cfg, s, c->timing && !stream)
                    : c->pd->enqueue(*cfg, s, c->timing && !stream);
     return set_last_error(rc);
@@ -117,6 +126,7 @@ extern "C" int mitos_ic_fetch_pk(mitos_ic_context* c, struct mitos_ic_result* re
     std::vector<long long> obs(bins);
     double* ds = c->pf ? c->pf->pk_sums : c->pd->pk_sums;
     long long* dob = c->pf ? c->pf->pk_obs : c->pd->pk_obs;
+    MITOS_TRY(set_last_error(wait_caller_stream(c)));
     MITOS_HIP_CHECK(hipStreamSynchronize(c->stream));
     MITOS_HIP_CHECK(hipMemcpy(sums.data(), ds, sizeof(double) * 2 * bins, hipMemcpyDeviceToHost));
     MITOS_HIP_CHECK(hipMemcpy(obs.data(), dob, sizeof(long long) * bins, hipMemcpyDeviceToHost));
@@ -204,6 +214,7 @@ extern "C" int mitos_ic_profile_entry(mitos_ic_context* c, int i, const char** n
 
 extern "C" int mitos_ic_positions(mitos_ic_context* c, float* X, float* Y, float* Z) {
     if (!c || !X || !Y || !Z) return set_last_error(MITOS_ERR_ARG);
+    MITOS_TRY(set_last_error(wait_caller_stream(c)));
     const size_t bytes = sizeof(float) * (size_t)c->N * c->N * c->N;
     float* src[3] = {c->pf ? c->pf->X : c->pd->X, c->pf ? c->pf->Y : c->pd->Y,
                      c->pf ? c->pf->Z : c->pd->Z};
@@ -272,6 +283,7 @@ static int unstencil_psi(Pipe<T>* P, int N, T* out, hipStream_t s) {
 
 extern "C" int mitos_ic_displacements(mitos_ic_context* c, void* psi) {
     if (!c || !psi) return set_last_error(MITOS_ERR_ARG);
+    MITOS_TRY(set_last_error(wait_caller_stream(c)));
     const long n = (long)c->N * c->N * c->N;
     if (c->pf && c->pf->psi_stencilled)
         MITOS_TRY(set_last_error(unstencil_psi(c->pf, c->N, (float*)psi, c->stream)));
@@ -290,6 +302,7 @@ extern "C" int mitos_ic_displacements(mitos_ic_context* c, void* psi) {
 
 extern "C" int mitos_ic_density(mitos_ic_context* c, void* delta) {
     if (!c || !delta) return set_last_error(MITOS_ERR_ARG);
+    MITOS_TRY(set_last_error(wait_caller_stream(c)));
     const long N = c->N;
     const long n = N * N * N;
     const double norm = ldexp(1.0, -(c->pf ? Pipe<float>::cic_shift() : Pipe<double>::cic_shift()));

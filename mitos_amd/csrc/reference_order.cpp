@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/mitos_hip.h"
+#include "runtime.h"
 
 namespace {
 
@@ -116,8 +117,10 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
         psi[i] = G.make(N, L, &rc);
         RTRY(rc);
         free_local_complex_grid(psi[i]);
-        if (hipMemcpy(psi[i]->box, derivative->box, sizeof(double) * N * (long)N * (N + 2),
-                      hipMemcpyDeviceToDevice) != hipSuccess)
+        // on the library stream, ahead of the next derivative into the same grid
+        if (hipMemcpyAsync(psi[i]->box, derivative->box, sizeof(double) * N * (long)N * (N + 2),
+                           hipMemcpyDeviceToDevice, mitos::api_stream()) != hipSuccess ||
+            mitos::api_sync() != MITOS_OK)
             return MITOS_ERR_DEVICE;
         psi[i]->momentum_space = 0;
     }
