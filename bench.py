@@ -34,6 +34,24 @@ def b_alg(N: int, s: int) -> float:
     return s * (28.0 * N ** 3 + 60.0 * N ** 2 * (N + 2))
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "ic_fused_fp32_1024_hbm_traffic.json")
+
+
+def measured_traffic(stage: str, N: int):
+    """HBM bytes per launch of a pipeline stage from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), or
+    None when the table does not cover this size and stage."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    st = t.get("stages", {}).get(stage)
+    if t.get("N") != N or not st:
+        return None
+    return st["traffic_bytes"] / st["launches"]
+
+
 def cpu_baseline(min_seconds: float = 10.0) -> dict:
     """The oracle's reference-order FP64 pipeline on one host core."""
     from oracle import oracle as O
@@ -185,7 +203,8 @@ def main():
     dname, (dms, dby, dn) = dom
     achieved = dby / (dms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                       "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                       "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                       "traffic": measured_traffic(dname, N),
                        "kernel": dname, "launches": dn, "kernel_ms": dms / dn,
                        "algorithmic_bytes_per_launch": dby / dn}
     out["kernels"] = {k: {"ms": v[0], "GB_per_s": v[1] / (v[0] * 1e-3) / 1e9 if v[0] else None,
