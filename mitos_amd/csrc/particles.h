@@ -10,10 +10,10 @@
 namespace mitos {
 
 // fixed-point unit of the CIC deposit: weights are summed as integers of
-// 2^-40 (2^-31 for float weights), so the total does not depend on the order
+// 2^-40 (2^-24 for float weights), so the total does not depend on the order
 // of the atomic adds
 #define MITOS_CIC_SHIFT 40
-#define MITOS_CIC_SHIFT_F32 31
+#define MITOS_CIC_SHIFT_F32 24
 
 __host__ __device__ __forceinline__ double fwrap_d(double x, double L) {
     return fmod(fmod(x, L) + L, L);  // include/distributed_grid.h:27
@@ -461,7 +461,8 @@ __device__ __forceinline__ float fwrap_float(float x, float Lf, double L) {
 
 // Float variant of cic_cell/cic_weights (FP32 pipeline): the offset in the
 // cell is formed as X - iX * cell with one rounding, so it keeps the float
-// position's accuracy; weights in units of 2^-31 (a weight of 1 still fits).
+// position's accuracy; weights in units of 2^-24 (the float mantissa), so a
+// 32-bit cell holds the mass of 256 particles.
 __device__ __forceinline__ void cic_cell_f(float X, float Y, float Z, int N, float cell,
                                            float cinv, double boxlen, int& iX, int& iY, int& iZ,
                                            float& dx, float& dy, float& dz) {
@@ -663,6 +664,17 @@ __device__ __forceinline__ void cic_global_near(unsigned long long* grid, int N,
     atomicAdd(grid + (r11 + z1), w[7]);
 }
 
+// Float cell of one coordinate of a position in [0, L] (the displacement
+// kernels write positions wrapped into the box): i = floor(x/cell) clamped
+// to [0, N] (N is cell 0 after the periodic offset below; the clamp keeps
+// any input inside the grid), d = x/cell - i with one rounding.
+__device__ __forceinline__ void cic_axis_f(float x, float cinv, int N, int& i, float& d) {
+    int k = (int)(x * cinv);
+    k = k < 0 ? 0 : (k > N ? N : k);
+    i = k;
+    d = fmaf(x, cinv, -(float)k);
+}
+
 // Cell of a lattice particle at site (qx, qy, qz) with position (x, y, z)
 // for the tiled deposit: base cell q + r with r the periodic cell offset in
 // [-N/2, N/2) (so the base index lies in [-N/2, 3N/2)), and the in-cell
@@ -673,10 +685,14 @@ __device__ __forceinline__ void cic_lattice_cell(float x, float y, float z, int 
                                                  int& rx, int& ry, int& rz, W& fx, W& fy,
                                                  W& fz) {
     int iX, iY, iZ;
-    if constexpr (sizeof(W) == 8)
+    if constexpr (sizeof(W) == 8) {
         cic_cell(x, y, z, N, boxlen, iX, iY, iZ, fx, fy, fz);
-    else
-        cic_cell_f(x, y, z, N, cell, cinv, boxlen, iX, iY, iZ, fx, fy, fz);
+    } else {
+        (void)cell;
+        cic_axis_f(x, cinv, N, iX, fx);
+        cic_axis_f(y, cinv, N, iY, fy);
+        cic_axis_f(z, cinv, N, iZ, fz);
+    }
     rx = iX - qx;
     ry = iY - qy;
     rz = iZ - qz;
@@ -685,8 +701,9 @@ __device__ __forceinline__ void cic_lattice_cell(float x, float y, float z, int 
     rz = rz < -N / 2 ? rz + N : (rz >= N / 2 ? rz - N : rz);
 }
 
-// LDS window word: 32 bits for float weights (2^-31 units: a carry out of a
-// cell is added to the grid as 2^32 at once), 64 bits for double weights
+// LDS window word: 32 bits for float weights (2^-24 units: a carry out of a
+// cell, past 256 particle masses, is added to the grid as 2^32 at once), 64
+// bits for double weights
 template <typename W>
 struct CicWin {
     typedef typename std::conditional<sizeof(W) == 4, unsigned, unsigned long long>::type type;
@@ -718,6 +735,36 @@ __device__ __noinline__ inline void cic_win_carry(unsigned long long* grid, unsi
                       1ull << 32);
 }
 
+// the carries of cic_win_carry appended to the carry list
+struct CicLists;
+__device__ __noinline__ inline void cic_list_carry(const CicLists& l, unsigned mask, int gx0,
+                                                   int gy0, int gz0, int ax, int ay, int az,
+                                                   int N);
+
+// Far particles and window carries of the two-pass deposit, for the third
+// pass (k_cic_far): lattice indices of particles whose cloud left their
+// tile's window, and grid cells owed 2^32 by a window cell that wrapped.
+// counts[0], counts[1]: entries of far and carry.
+struct CicLists {
+    unsigned* far;
+    unsigned* carry;
+    unsigned* counts;
+    unsigned carry_cap;
+};
+
+__device__ __noinline__ inline void cic_list_carry(const CicLists& l, unsigned mask, int gx0,
+                                                   int gy0, int gz0, int ax, int ay, int az,
+                                                   int N) {
+    const int cx[8] = {0, 1, 0, 0, 1, 1, 0, 1}, cy[8] = {0, 0, 1, 0, 1, 0, 1, 1},
+              cz[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+    for (int c = 0; c < 8; c++)
+        if (mask & (1u << c)) {
+            const unsigned k = atomicAdd(l.counts + 1, 1u);
+            if (k < l.carry_cap)  // the capacity bounds every carry a deposit can make
+                l.carry[k] = cic_win_cell(gx0, gy0, gz0, ax + cx[c], ay + cy[c], az + cz[c], N);
+        }
+}
+
 // CIC deposit (analyse/src/mitos_read.c:281-334) of particles stored in
 // lattice order (X[(qx*N + qy)*N + qz] belongs to lattice site q).  Each
 // work-group loops over PX*PY*PZ tiles of lattice sites (tile = blockIdx.x +
@@ -730,24 +777,30 @@ __device__ __noinline__ inline void cic_win_carry(unsigned long long* grid, unsi
 // atomics after the flush; the flush adds every non-zero window cell to the
 // grid with one atomic and clears it.
 // W = double: the reference's double weights in units of 2^-40 (FP64
-// pipeline), 64-bit window; W = float: float weights in units of 2^-31 (FP32
+// pipeline), 64-bit window; W = float: float weights in units of 2^-24 (FP32
 // pipeline), 32-bit window with carries.
-template <typename W, int PX, int PY, int PZ, int RX, int RY, int RZ, int NT>
+template <typename W, int PX, int PY, int PZ, int RX, int RY, int RZ, int NT,
+          bool STREAM = false>
 __global__ void __launch_bounds__(NT)
     k_cic_lattice(const float* __restrict__ X, const float* __restrict__ Y,
                   const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid,
-                  int ntiles) {
+                  int ntiles, typename CicWin<W>::type* __restrict__ scratch = nullptr,
+                  int* __restrict__ means = nullptr, int* __restrict__ mmax = nullptr,
+                  CicLists lists = CicLists{}) {
     typedef typename CicWin<W>::type WT;
+    static_assert(!STREAM || (sizeof(WT) == 4 && (RX * RY * RZ) % 4 == 0),
+                  "streamed windows are 32-bit words written 16 bytes at a time");
     constexpr int P = PX * PY * PZ, K = P / NT, R = RX * RY * RZ, QCAP = 512;
     static_assert(P % NT == 0 && NT % 64 == 0, "tile must be a multiple of the work-group");
     static_assert(P <= 65536, "queue entries are 16-bit tile slots");
     // the flush wraps window cells with one (x, y) or two (z) subtractions of
     // N, so the window fits in N = 32 (the smallest grid this kernel runs on)
     static_assert(RX <= 32 && RY <= 32 && RZ <= 64, "window too large for N >= 32");
-    __shared__ WT win[R];
+    __shared__ __align__(16) WT win[R];
     __shared__ int red[3][NT / 64];
     __shared__ unsigned short queue[QCAP];
     __shared__ int qn;
+    __shared__ unsigned far_base;
     const int ntz = N / PZ, nty = N / PY;
     const float cell = (float)(boxlen / N), cinv = (float)(N / boxlen);
     const int lane = threadIdx.x & 63;
@@ -771,19 +824,20 @@ __global__ void __launch_bounds__(NT)
     if ((int)blockIdx.x < ntiles) load(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-        // mean cell offset of the tile (cells are recomputed for the deposit)
+        // cells of the tile's particles and their mean offset
+        int rxs[K], rys[K], rzs[K];
+        W fx[K], fy[K], fz[K];
         int sx = 0, sy = 0, sz = 0;
 #pragma unroll
         for (int i = 0; i < K; i++) {
             const int p = threadIdx.x + NT * i;
             const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
-            int rx, ry, rz;
-            W dx, dy, dz;
             cic_lattice_cell(px[i], py[i], pz[i], N, boxlen, cell, cinv, tx * PX + lx,
-                             ty * PY + ly, tz * PZ + lz, rx, ry, rz, dx, dy, dz);
-            sx += rx;
-            sy += ry;
-            sz += rz;
+                             ty * PY + ly, tz * PZ + lz, rxs[i], rys[i], rzs[i], fx[i], fy[i],
+                             fz[i]);
+            sx += rxs[i];
+            sy += rys[i];
+            sz += rzs[i];
         }
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
@@ -812,12 +866,9 @@ __global__ void __launch_bounds__(NT)
             const int p = threadIdx.x + NT * i;
             const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
             const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
-            int rx, ry, rz;
-            W dx, dy, dz;
-            cic_lattice_cell(px[i], py[i], pz[i], N, boxlen, cell, cinv, qx, qy, qz, rx, ry, rz,
-                             dx, dy, dz);
+            const int rx = rxs[i], ry = rys[i], rz = rzs[i];
             unsigned long long w[8];
-            cic_weights(dx, dy, dz, w);
+            cic_weights(fx[i], fy[i], fz[i], w);
             const int ax = qx + rx - ox, ay = qy + ry - oy, az = qz + rz - oz;
             const bool in = ax >= 0 && ax + 1 < RX && ay >= 0 && ay + 1 < RY && az >= 0 &&
                             az + 1 < RZ;
@@ -826,13 +877,27 @@ __global__ void __launch_bounds__(NT)
                 constexpr int off[8] = {0, RY * RZ, RZ, 1, RY * RZ + RZ, RY * RZ + 1, RZ + 1,
                                         RY * RZ + RZ + 1};
                 if constexpr (sizeof(WT) == 4) {
-                    unsigned carry = 0;
+                    // a carry needs an old value of at least 2^31 (a weight
+                    // is at most 2^24): test that first, the exact carries
+                    // only then
+                    unsigned old[8], hi = 0;
 #pragma unroll
                     for (int c = 0; c < 8; c++) {
-                        const unsigned old = atomicAdd(b + off[c], (unsigned)w[c]);
-                        carry |= (old + (unsigned)w[c] < old) ? (1u << c) : 0u;
+                        old[c] = atomicAdd(b + off[c], (unsigned)w[c]);
+                        hi |= old[c];
                     }
-                    if (carry) cic_win_carry(grid, carry, gx0, gy0, gz0, ax, ay, az, N);
+                    if (hi >> 31) {
+                        unsigned carry = 0;
+#pragma unroll
+                        for (int c = 0; c < 8; c++)
+                            carry |= (old[c] + (unsigned)w[c] < old[c]) ? (1u << c) : 0u;
+                        if (carry) {
+                            if constexpr (STREAM)
+                                cic_list_carry(lists, carry, gx0, gy0, gz0, ax, ay, az, N);
+                            else
+                                cic_win_carry(grid, carry, gx0, gy0, gz0, ax, ay, az, N);
+                        }
+                    }
                 } else {
 #pragma unroll
                     for (int c = 0; c < 8; c++) atomicAdd(b + off[c], (WT)w[c]);
@@ -846,42 +911,78 @@ __global__ void __launch_bounds__(NT)
                 base = __shfl(base, lead, 64);
                 if (!in) {
                     const int slot = base + __popcll(m & ((1ull << lane) - 1));
-                    if (slot < QCAP)
+                    if (slot < QCAP) {
                         queue[slot] = (unsigned short)(threadIdx.x + NT * i);
-                    else
+                    } else if constexpr (STREAM) {
+                        lists.far[atomicAdd(lists.counts, 1u)] =
+                            ((unsigned)qx * N + (unsigned)qy) * N + (unsigned)qz;
+                    } else {
                         cic_global_near(grid, N, qx + rx, qy + ry, qz + rz, w);
+                    }
                 }
             }
         }
         __syncthreads();
         // the next tile's positions load while this window is flushed
         if (tile + (int)gridDim.x < ntiles) load(tile + gridDim.x);
-        // window cells to the grid and cleared; (cx, cy, cz) of cell c
-        // advance with c += NT
-        constexpr int DZ = NT % RZ, DY = (NT / RZ) % RY, DX = NT / (RZ * RY);
-        int cz = threadIdx.x % RZ, cy = (threadIdx.x / RZ) % RY, cx = threadIdx.x / (RZ * RY);
-        for (int c = threadIdx.x; c < R; c += NT) {
-            const WT v = win[c];
-            if (v) {
-                win[c] = 0;
-                atomicAdd(grid + cic_win_cell(gx0, gy0, gz0, cx, cy, cz, N),
-                          (unsigned long long)v);
+        if constexpr (STREAM) {
+            // the whole window to the tile's scratch slot (plain 16-byte
+            // stores) and its mean offset to the tile table; k_cic_gather
+            // adds the windows to the grid
+            if (threadIdx.x == 0) {
+                const int m[3] = {ox - tx * PX + (RX - PX - 1) / 2, oy - ty * PY + (RY - PY - 1) / 2,
+                                  oz - tz * PZ + (RZ - PZ - 1) / 2};
+                for (int a = 0; a < 3; a++) {
+                    means[3 * tile + a] = m[a];
+                    atomicMax(mmax + a, m[a] < 0 ? -m[a] : m[a]);
+                }
             }
-            cz += DZ;
-            cy += DY;
-            cx += DX;
-            if (cz >= RZ) {
-                cz -= RZ;
-                cy++;
+            uint4* dst = reinterpret_cast<uint4*>(scratch + (size_t)tile * R);
+            uint4* src = reinterpret_cast<uint4*>(win);
+            for (int c = threadIdx.x; c < R / 4; c += NT) {
+                dst[c] = src[c];
+                src[c] = make_uint4(0, 0, 0, 0);
             }
-            if (cy >= RY) {
-                cy -= RY;
-                cx++;
+        } else {
+            // window cells to the grid and cleared; (cx, cy, cz) of cell c
+            // advance with c += NT
+            constexpr int DZ = NT % RZ, DY = (NT / RZ) % RY, DX = NT / (RZ * RY);
+            int cz = threadIdx.x % RZ, cy = (threadIdx.x / RZ) % RY,
+                cx = threadIdx.x / (RZ * RY);
+            for (int c = threadIdx.x; c < R; c += NT) {
+                const WT v = win[c];
+                if (v) {
+                    win[c] = 0;
+                    atomicAdd(grid + cic_win_cell(gx0, gy0, gz0, cx, cy, cz, N),
+                              (unsigned long long)v);
+                }
+                cz += DZ;
+                cy += DY;
+                cx += DX;
+                if (cz >= RZ) {
+                    cz -= RZ;
+                    cy++;
+                }
+                if (cy >= RY) {
+                    cy -= RY;
+                    cx++;
+                }
             }
         }
-        // queued particles: the same cell and weights, recomputed
+        // queued particles: to the far list (two passes), or deposited now
+        // with the same cell and weights, recomputed
         const int nq = qn < QCAP ? qn : QCAP;
-        for (int j = threadIdx.x; j < nq; j += NT) {
+        if constexpr (STREAM) {
+            if (threadIdx.x == 0 && nq) far_base = atomicAdd(lists.counts, (unsigned)nq);
+            __syncthreads();
+            for (int j = threadIdx.x; j < nq; j += NT) {
+                const int p = queue[j];
+                const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
+                lists.far[far_base + j] =
+                    ((unsigned)(tx * PX + lx) * N + (unsigned)(ty * PY + ly)) * N + tz * PZ + lz;
+            }
+        }
+        for (int j = threadIdx.x; j < (STREAM ? 0 : nq); j += NT) {
             const int p = queue[j];
             const int lz = p % PZ, ly = (p / PZ) % PY, lx = p / (PZ * PY);
             const int qx = tx * PX + lx, qy = ty * PY + ly, qz = tz * PZ + lz;
@@ -913,19 +1014,184 @@ struct CicTile<double> {
     static constexpr int PX = 8, PY = 16, PZ = 16, RX = 20, RY = 28, RZ = 28, NT = 512;
 };
 
+// floor(a / b) for b > 0
+__device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Second pass of the two-pass lattice deposit: one work-group per output
+// tile of PX*PY*PZ cells (the cells of the lattice tile of the same index).
+// Every window that can meet the tile (tile offsets bounded by the largest
+// mean offset, mmax) is listed in LDS, and each thread sums its (y, z) column
+// of the tile over those windows in registers; the column is then stored to
+// the grid (every grid cell lies in exactly one output tile).  Each window
+// cell lies in one output tile, and with R + P <= N per axis only one
+// periodic image of a window meets it, so every cell is counted once; the
+// sums are integers, so the order of the windows does not matter.
+template <int PX, int PY, int PZ, int RX, int RY, int RZ, int NT>
+__global__ void __launch_bounds__(NT)
+    k_cic_gather(const unsigned* __restrict__ scratch, const int* __restrict__ means,
+                 const int* __restrict__ mmax, int N, unsigned long long* __restrict__ grid) {
+    static_assert(NT == PY * PZ, "one thread per (y, z) column of the tile");
+    constexpr int R = RX * RY * RZ, PLANE = RY * RZ, MAXC = 256;
+    constexpr int HX = (RX - PX - 1) / 2, HY = (RY - PY - 1) / 2, HZ = (RZ - PZ - 1) / 2;
+    __shared__ int cand[MAXC][4];  // source tile and window origin relative to this tile
+    __shared__ int ncand;
+    const int ntz = N / PZ, nty = N / PY, ntx = N / PX;
+    const int tile = blockIdx.x;
+    const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+    const int yt = threadIdx.x / PZ, zt = threadIdx.x % PZ;
+    unsigned long long acc[PX];
+#pragma unroll
+    for (int x = 0; x < PX; x++) acc[x] = 0;
+    const int MX = mmax[0], MY = mmax[1], MZ = mmax[2];
+    // source tiles d with a window [d P + m - h, d P + m - h + R) that can
+    // meet [0, P), |m| <= M; those that do are listed in LDS
+    const int dx0 = floor_div(HX - MX - RX, PX) + 1, dx1 = floor_div(PX + HX + MX - 1, PX);
+    const int dy0 = floor_div(HY - MY - RY, PY) + 1, dy1 = floor_div(PY + HY + MY - 1, PY);
+    const int dz0 = floor_div(HZ - MZ - RZ, PZ) + 1, dz1 = floor_div(PZ + HZ + MZ - 1, PZ);
+    const int nx = dx1 - dx0 + 1, ny = dy1 - dy0 + 1, nz = dz1 - dz0 + 1;
+    const int ntot = nx * ny * nz;
+    // candidates in chunks of MAXC, so the list never overflows
+    for (int base = 0; base < ntot; base += MAXC) {
+        if (threadIdx.x == 0) ncand = 0;
+        __syncthreads();
+        const int cend = base + MAXC < ntot ? base + MAXC : ntot;
+        for (int c = base + threadIdx.x; c < cend; c += NT) {
+            const int dz = dz0 + c % nz, dy = dy0 + (c / nz) % ny, dx = dx0 + c / (nz * ny);
+            const int sx = ((tx + dx) % ntx + ntx) % ntx, sy = ((ty + dy) % nty + nty) % nty,
+                      sz = ((tz + dz) % ntz + ntz) % ntz;
+            const int src = (sx * nty + sy) * ntz + sz;
+            const int ox = dx * PX + means[3 * src + 0] - HX;
+            const int oy = dy * PY + means[3 * src + 1] - HY;
+            const int oz = dz * PZ + means[3 * src + 2] - HZ;
+            if (ox < PX && ox + RX > 0 && oy < PY && oy + RY > 0 && oz < PZ && oz + RZ > 0) {
+                const int k = atomicAdd(&ncand, 1);
+                cand[k][0] = src;
+                cand[k][1] = ox;
+                cand[k][2] = oy;
+                cand[k][3] = oz;
+            }
+        }
+        __syncthreads();
+        const int nc = ncand;
+        for (int k = 0; k < nc; k++) {
+            const int src = cand[k][0], ox = cand[k][1], oy = cand[k][2], oz = cand[k][3];
+            const bool in = yt >= oy && yt < oy + RY && zt >= oz && zt < oz + RZ;
+            const unsigned* w = scratch + (size_t)src * R +
+                                (in ? (yt - oy) * RZ + (zt - oz) : 0) - (ptrdiff_t)ox * PLANE;
+            // the column's loads first (x outside the window skipped as a
+            // group), then the sums
+            unsigned v[PX];
+#pragma unroll
+            for (int x = 0; x < PX; x++) {
+                v[x] = 0;
+                if (x >= ox && x < ox + RX && in) v[x] = w[(ptrdiff_t)x * PLANE];
+            }
+#pragma unroll
+            for (int x = 0; x < PX; x++) acc[x] += v[x];
+        }
+        __syncthreads();
+    }
+    const unsigned n = (unsigned)N;
+    unsigned long long* g = grid + ((unsigned)(tx * PX) * n + (unsigned)(ty * PY + yt)) * n +
+                            (unsigned)(tz * PZ + zt);
+#pragma unroll
+    for (int x = 0; x < PX; x++) g[(size_t)x * n * n] = acc[x];
+}
+
+// Third pass: far particles and window carries added to the grid.
+template <typename W>
+__global__ void __launch_bounds__(256)
+    k_cic_far(const float* __restrict__ X, const float* __restrict__ Y,
+              const float* __restrict__ Z, int N, double boxlen, unsigned long long* grid,
+              CicLists lists) {
+    const float cell = (float)(boxlen / N), cinv = (float)(N / boxlen);
+    const unsigned nf = lists.counts[0];
+    const unsigned nc = lists.counts[1] < lists.carry_cap ? lists.counts[1] : lists.carry_cap;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < nf; j += stride) {
+        const unsigned idx = lists.far[j];
+        const int qz = (int)(idx % (unsigned)N), qy = (int)((idx / (unsigned)N) % (unsigned)N),
+                  qx = (int)(idx / ((unsigned)N * (unsigned)N));
+        int rx, ry, rz;
+        W dx, dy, dz;
+        cic_lattice_cell(X[idx], Y[idx], Z[idx], N, boxlen, cell, cinv, qx, qy, qz, rx, ry, rz,
+                         dx, dy, dz);
+        unsigned long long w[8];
+        cic_weights(dx, dy, dz, w);
+        cic_global_near(grid, N, qx + rx, qy + ry, qz + rz, w);
+    }
+    for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += stride)
+        atomicAdd(grid + lists.carry[j], 1ull << 32);
+}
+
 // the tile kernel of window type W
-template <typename W, typename C = CicTile<W>>
+template <typename W, bool STREAM = false, typename C = CicTile<W>>
 constexpr auto cic_lattice_kernel() {
-    return k_cic_lattice<W, C::PX, C::PY, C::PZ, C::RX, C::RY, C::RZ, C::NT>;
+    return k_cic_lattice<W, C::PX, C::PY, C::PZ, C::RX, C::RY, C::RZ, C::NT, STREAM>;
+}
+
+static __global__ void k_zero_i32(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Workspace of the two-pass deposit (float weights, N >= 128: a window slot
+// per tile, the tile mean offsets, their maxima and the list counts), in
+// bytes; 0 where the two-pass deposit does not apply.
+template <typename W>
+inline size_t cic_stream_bytes(int N) {
+    typedef CicTile<W> C;
+    if (sizeof(W) != 4 || N < 128 || N % 32 != 0) return 0;
+    const size_t ntiles = (size_t)(N / C::PX) * (N / C::PY) * (N / C::PZ);
+    return ntiles * C::RX * C::RY * C::RZ * sizeof(unsigned) + (ntiles * 3 + 8) * sizeof(int);
+}
+
+// capacity of the carry list: a tile's window holds at most P * 2^24 units
+// of mass, so it wraps at most P / 256 times
+template <typename W>
+inline size_t cic_carry_cap(int N) {
+    typedef CicTile<W> C;
+    return (size_t)N * N * N / 256 + 64;
+}
+
+// bytes of the far-particle and carry lists (every particle at most once)
+template <typename W>
+inline size_t cic_list_bytes(int N) {
+    return ((size_t)N * N * N + cic_carry_cap<W>(N)) * sizeof(unsigned);
+}
+
+template <typename W>
+inline int* cic_ws_meta(int N, void* ws) {
+    typedef CicTile<W> C;
+    const size_t ntiles = (size_t)(N / C::PX) * (N / C::PY) * (N / C::PZ);
+    return reinterpret_cast<int*>(reinterpret_cast<unsigned*>(ws) +
+                                  ntiles * C::RX * C::RY * C::RZ) +
+           ntiles * 3;
+}
+
+template <typename W>
+inline CicLists cic_lists(int N, void* ws, void* lists_mem) {
+    CicLists l;
+    l.far = reinterpret_cast<unsigned*>(lists_mem);
+    l.carry = l.far + (size_t)N * N * N;
+    l.counts = reinterpret_cast<unsigned*>(cic_ws_meta<W>(N, ws) + 4);
+    l.carry_cap = (unsigned)cic_carry_cap<W>(N);
+    return l;
 }
 
 // Launch the lattice deposit: as many work-groups as are resident at once
-// (per device and weight type), at most one per tile.
-template <typename W>
+// (per device, weight type and mode), at most one per tile.  STREAM (two
+// passes, with ws of cic_stream_bytes(N) and lists_mem of cic_list_bytes(N)):
+// the windows go to ws, far particles and carries to the lists, and
+// cic_gather_launch then cic_far_launch write the grid, which need not be
+// cleared.  Otherwise the windows are added to a cleared grid with atomics.
+template <typename W, bool STREAM>
 inline int cic_lattice_launch(const float* X, const float* Y, const float* Z, int N,
-                              double boxlen, unsigned long long* grid, hipStream_t s) {
+                              double boxlen, unsigned long long* grid, void* ws,
+                              void* lists_mem, hipStream_t s) {
     typedef CicTile<W> C;
     if (N % 32 != 0) return MITOS_ERR_UNSUPPORTED;
+    if (STREAM && (!ws || !lists_mem || cic_stream_bytes<W>(N) == 0)) return MITOS_ERR_ARG;
     const int ntiles = (N / C::PX) * (N / C::PY) * (N / C::PZ);
     static std::mutex mu;
     static std::map<int, int> resident;
@@ -940,14 +1206,52 @@ inline int cic_lattice_launch(const float* X, const float* Y, const float* Z, in
             MITOS_HIP_CHECK(
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             MITOS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, cic_lattice_kernel<W>(), C::NT, 0));
+                &per, cic_lattice_kernel<W, STREAM>(), C::NT, 0));
             if (cus <= 0 || per <= 0) return MITOS_ERR_DEVICE;
             it = resident.emplace(dev, cus * per).first;
         }
         groups = it->second < ntiles ? it->second : ntiles;
     }
-    hipLaunchKernelGGL(cic_lattice_kernel<W>(), dim3(groups), dim3(C::NT), 0, s, X, Y, Z, N,
-                       boxlen, grid, ntiles);
+    typedef typename CicWin<W>::type WT;
+    WT* scratch = nullptr;
+    int* means = nullptr;
+    int* meta = nullptr;
+    CicLists lists{};
+    if (STREAM) {
+        scratch = reinterpret_cast<WT*>(ws);
+        means = reinterpret_cast<int*>(scratch + (size_t)ntiles * C::RX * C::RY * C::RZ);
+        meta = cic_ws_meta<W>(N, ws);
+        lists = cic_lists<W>(N, ws, lists_mem);
+        // mean-offset maxima (meta[0..2]) and list counts (meta[4..5]) start at 0
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, meta, 8);
+    }
+    hipLaunchKernelGGL((cic_lattice_kernel<W, STREAM>()), dim3(groups), dim3(C::NT), 0, s, X, Y,
+                       Z, N, boxlen, grid, ntiles, scratch, means, meta, lists);
+    return MITOS_OK;
+}
+
+// second pass of the two-pass deposit: windows summed into the grid
+template <typename W>
+inline int cic_gather_launch(int N, unsigned long long* grid, void* ws, hipStream_t s) {
+    typedef CicTile<W> C;
+    if (!ws || cic_stream_bytes<W>(N) == 0) return MITOS_ERR_ARG;
+    const int ntiles = (N / C::PX) * (N / C::PY) * (N / C::PZ);
+    const unsigned* scratch = reinterpret_cast<const unsigned*>(ws);
+    const int* means =
+        reinterpret_cast<const int*>(scratch + (size_t)ntiles * C::RX * C::RY * C::RZ);
+    hipLaunchKernelGGL((k_cic_gather<C::PX, C::PY, C::PZ, C::RX, C::RY, C::RZ, C::NT>),
+                       dim3(ntiles), dim3(C::NT), 0, s, scratch, means, cic_ws_meta<W>(N, ws), N,
+                       grid);
+    return MITOS_OK;
+}
+
+// third pass of the two-pass deposit: far particles and carries
+template <typename W>
+inline int cic_far_launch(const float* X, const float* Y, const float* Z, int N, double boxlen,
+                          unsigned long long* grid, void* ws, void* lists_mem, hipStream_t s) {
+    if (!ws || !lists_mem || cic_stream_bytes<W>(N) == 0) return MITOS_ERR_ARG;
+    hipLaunchKernelGGL((k_cic_far<W>), dim3(1024), dim3(256), 0, s, X, Y, Z, N, boxlen, grid,
+                       cic_lists<W>(N, ws, lists_mem));
     return MITOS_OK;
 }
 

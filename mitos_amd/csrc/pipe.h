@@ -318,6 +318,9 @@ struct Pipe {
 
     FftSchedule sched;
     bool psi_stencilled = false;  // the displacement grids hold the stencil form
+    // workspace of the two-pass lattice deposit (float weights, N >= 128);
+    // null: the deposit adds its windows with atomics
+    void* cic_ws = nullptr;
 
     // time every schedule variant of the plain passes (or reuse the choice
     // made for this device, N and precision)
@@ -447,6 +450,14 @@ struct Pipe {
         MITOS_HIP_CHECK(hipMalloc(&X, sizeof(float) * np));
         MITOS_HIP_CHECK(hipMalloc(&Y, sizeof(float) * np));
         MITOS_HIP_CHECK(hipMalloc(&Z, sizeof(float) * np));
+        typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
+        const size_t wsb = cic_stream_bytes<M>(N);
+        const char* atomic_env = getenv("MITOS_CIC_ATOMIC");
+        const bool force_atomic = atomic_env && atomic_env[0] == '1';
+        if (wsb && !force_atomic && hipMalloc(&cic_ws, wsb) != hipSuccess) {
+            (void)hipGetLastError();  // no room: the atomic deposit needs none
+            cic_ws = nullptr;
+        }
         for (int i = 0; i < 9; i++) MITOS_HIP_CHECK(hipEventCreate(&ev[i]));
         for (int i = 0; i < 33; i++) MITOS_HIP_CHECK(hipEventCreate(&pev[i]));
         return MITOS_OK;
@@ -464,6 +475,7 @@ struct Pipe {
         hipFree(pk_obs);
         hipFree(pk_part);
         hipFree(pk_cnt);
+        hipFree(cic_ws);
         for (int i = 0; i < 9; i++)
             if (ev[i]) hipEventDestroy(ev[i]);
         for (int i = 0; i < 33; i++)
@@ -643,14 +655,20 @@ struct Pipe {
             mark(4);
             // 9-10. lattice + sequential TSC displacement + wrap + CIC deposit
             const float spacing = (float)L / NN;
-            MITOS_TRY(timed(s, "cic_clear", 8.0 * np, [&] {
-                hipLaunchKernelGGL((k_fill_u64<T>), dim3(nblk(np)), dim3(256), 0, s, cic(),
-                                   (long)np);
-                return launch_status();
-            }));
+            // the two-pass deposit of the lattice particles stores every
+            // grid cell, the one-pass and the adjoint-mode deposits add to a
+            // cleared grid (far-particle lists use K, free after combine_x)
+            typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
+            const bool two_pass = !noise_in && cic_ws && NN % 32 == 0 &&
+                                  cic_list_bytes<M>(NN) <= sizeof(cx<T>) * (size_t)csize;
+            if (!two_pass)
+                MITOS_TRY(timed(s, "cic_clear", 8.0 * np, [&] {
+                    hipLaunchKernelGGL((k_fill_u64<T>), dim3(nblk(np)), dim3(256), 0, s, cic(),
+                                       (long)np);
+                    return launch_status();
+                }));
             mark(5);
             // FP32 moves the particles in float arithmetic, FP64 in double
-            typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
             if (noise_in) {
                 // the forward of the adjoint: positions stay in T up to the deposit
                 MITOS_TRY(timed(s, "move_deposit", 3 * Rg + 12.0 * np + 8.0 * np, [&] {
@@ -680,16 +698,37 @@ struct Pipe {
                     }
                     return launch_status();
                 }));
-                // reads 3 float coordinates, writes the grid
-                MITOS_TRY(timed(s, "cic_deposit", 12.0 * np + 8.0 * np, [&] {
+                // reads 3 float coordinates, writes the grid (two passes: the
+                // windows through the workspace, then summed into the grid)
+                MITOS_TRY(timed(s, "cic_deposit", two_pass ? 12.0 * np : 20.0 * np, [&] {
+                    if constexpr (sizeof(M) == 4) {
+                        if (two_pass) {
+                            MITOS_TRY((cic_lattice_launch<M, true>(X, Y, Z, NN, L, cic(), cic_ws,
+                                                                   K, s)));
+                            return launch_status();
+                        }
+                    }
                     if (NN % 32 == 0) {
-                        MITOS_TRY(cic_lattice_launch<M>(X, Y, Z, NN, L, cic(), s));
+                        MITOS_TRY((cic_lattice_launch<M, false>(X, Y, Z, NN, L, cic(), nullptr,
+                                                                nullptr, s)));
                         return launch_status();
                     }
                     hipLaunchKernelGGL(k_cic<M>, dim3(nblk(np)), dim3(256), 0, s, X, Y, Z, np, NN, L,
                                        cic());
                     return launch_status();
                 }));
+                if constexpr (sizeof(M) == 4) {
+                    if (two_pass) {
+                        MITOS_TRY(timed(s, "cic_gather", 8.0 * np, [&] {
+                            MITOS_TRY(cic_gather_launch<M>(NN, cic(), cic_ws, s));
+                            return launch_status();
+                        }));
+                        MITOS_TRY(timed(s, "cic_far", 0.0, [&] {
+                            MITOS_TRY(cic_far_launch<M>(X, Y, Z, NN, L, cic(), cic_ws, K, s));
+                            return launch_status();
+                        }));
+                    }
+                }
             }
             mark(6);
             // 11. density r2c (overdensity on load), 12. P(k) with CIC window
