@@ -35,6 +35,17 @@ __host__ __device__ __forceinline__ double fwrap_fast(double x, double L) {
     return r;
 }
 
+__device__ __noinline__ inline float fwrap_float_far(float x, double L) {
+    return (float)fwrap_fast(x, L);
+}
+
+// Float periodic wrap of a float position: x itself on [0, Lf), Lf = (float)L
+// (positions written by the displacement kernels lie there), the double wrap
+// otherwise.
+__device__ __forceinline__ float fwrap_float(float x, float Lf, double L) {
+    return (x >= 0.0f && x < Lf) ? x : fwrap_float_far(x, L);
+}
+
 // one axis of the TSC window (src/grids_interp.c:291-296): the three taps with
 // non-zero weight, in the reference's loop order.  X is in cell units.
 template <typename M>
@@ -338,7 +349,7 @@ __global__ void __launch_bounds__(NT)
                          const T* __restrict__ sz, unsigned rs, int N, int nch, T Lf,
                          float spacing, double boxlen, float* __restrict__ X,
                          float* __restrict__ Y, float* __restrict__ Z) {
-    static_assert(TX % PPT == 0, "a thread's rows lie in one strip");
+    static_assert(TX == 1 || TX % PPT == 0, "a thread's rows lie in one strip");
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned rowg = bid / (unsigned)nch;
     const int qz = (int)(bid - rowg * (unsigned)nch) * NT + threadIdx.x;
@@ -352,8 +363,11 @@ __global__ void __launch_bounds__(NT)
     float x[PPT], y[PPT], z[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
+        // TX = 1: PPT consecutive rows along y (the taps of a work-group then
+        // stay in a few x-planes); TX > 1: strips of TX x-planes walked row
+        // by row in y
         const unsigned rowi = rowg * PPT + j;
-        const unsigned xi = rowi % TX, r2 = rowi / TX;
+        const unsigned xi = TX > 1 ? rowi % TX : 0, r2 = TX > 1 ? rowi / TX : rowi;
         qy[j] = (int)(r2 % (unsigned)N);
         qx[j] = (int)(r2 / (unsigned)N) * TX + (int)xi;
         yz[j] = (unsigned)qy[j] * rs + zo;
@@ -409,9 +423,177 @@ __global__ void __launch_bounds__(NT)
         }
         z[j] = (float)((T)z[j] - d);
         const long long p = ((long long)qx[j] * N + qy[j]) * N + qz;
-        X[p] = (float)fwrap_fast(x[j], boxlen);
-        Y[p] = (float)fwrap_fast(y[j], boxlen);
-        Z[p] = (float)fwrap_fast(z[j], boxlen);
+        const float Lp = (float)Lf;
+        X[p] = fwrap_float(x[j], Lp, boxlen);
+        Y[p] = fwrap_float(y[j], Lp, boxlen);
+        Z[p] = fwrap_float(z[j], Lp, boxlen);
+    }
+}
+
+// wave-wide minimum and maximum of an int
+__device__ __forceinline__ void wave_minmax(int& lo, int& hi) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+}
+
+// k_displace_lattice_s with the taps staged in LDS.  One wave per work-group
+// moves PPT x-consecutive lattice rows (one y) over 64 consecutive z.  After
+// the x move the wave's x taps span a few rows [lo - 1, hi + 1]: those rows of
+// the y displacement grid (the 64 z of the wave) are loaded whole, as
+// coalesced 256-byte rows, into LDS, and the y move reads its taps there;
+// the rows (x taps) x (y taps) of the z displacement grid follow after the y
+// move.  A wave whose taps span more than MAXX x rows or MAXY y rows reads
+// them from the grids directly.  Weights, taps and the order of the sums are
+// those of k_displace_lattice_s, so the positions are the same bits.
+// N must be a multiple of 64; `staged` = false reads every tap directly.
+template <typename T, int TX, int PPT, int MAXX, int MAXY>
+__global__ void __launch_bounds__(64)
+    k_displace_lattice_w(const T* __restrict__ sx, const T* __restrict__ sy,
+                         const T* __restrict__ sz, unsigned rs, int N, T Lf, float spacing,
+                         double boxlen, float* __restrict__ X, float* __restrict__ Y,
+                         float* __restrict__ Z, int staged) {
+    static_assert(TX % PPT == 0, "a thread's rows lie in one strip");
+    __shared__ T buf[MAXX * MAXY][64];
+    const int nch = N / 64;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned rowg = bid / (unsigned)nch;
+    const int lane = threadIdx.x;
+    const int qz = (int)(bid - rowg * (unsigned)nch) * 64 + lane;
+    const float invN = 1.0f / N;
+    const T Nf = (T)N;
+    const unsigned plane = (unsigned)N * rs;
+    const unsigned zo = MITOS_PSI_HO + qz;
+    int qx[PPT];
+    float x[PPT], y[PPT], z[PPT];
+    const unsigned row0 = rowg * PPT;
+    const unsigned r2 = row0 / TX;
+    const int qy = (int)(r2 % (unsigned)N);
+    const unsigned yz = (unsigned)qy * rs + zo;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        qx[j] = (int)(r2 / (unsigned)N) * TX + (int)((row0 + j) % TX);
+        // src/particle.c:107-109
+        x[j] = qx[j] * spacing;
+        y[j] = qy * spacing;
+        z[j] = qz * spacing;
+    }
+    // x move: the stencilled x displacement at the site
+    T sxv[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) sxv[j] = sx[(unsigned)qx[j] * plane + yz];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) x[j] = (float)((T)x[j] - sxv[j]);
+    // y move: TSC along x at (x', q_y, q_z)
+    T wx[PPT][3];
+    int cx[PPT];
+    int lo = 1 << 30, hi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        tsc_axis_near((T)x[j] * Nf / Lf, cx[j], wx[j]);
+        lo = cx[j] < lo ? cx[j] : lo;
+        hi = cx[j] > hi ? cx[j] : hi;
+    }
+    wave_minmax(lo, hi);
+    const int nx = hi - lo + 3;  // x rows lo - 1 .. hi + 1
+    const bool stage_x = staged && nx <= MAXX;
+    T syv[PPT][3];
+    if (stage_x) {
+        T v[MAXX];
+#pragma unroll
+        for (int i = 0; i < MAXX; i++)
+            if (i < nx) v[i] = sy[(unsigned)wrapf(lo - 1 + i, N, invN) * plane + yz];
+#pragma unroll
+        for (int i = 0; i < MAXX; i++)
+            if (i < nx) buf[i][lane] = v[i];
+#pragma unroll
+        for (int j = 0; j < PPT; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) syv[j][a] = buf[cx[j] - lo + a][lane];
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            unsigned o[3];
+            tap_rows(wrapf(cx[j], N, invN), N, plane, o);
+#pragma unroll
+            for (int a = 0; a < 3; a++) syv[j][a] = sy[o[a] + yz];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++)
+        y[j] = (float)((T)y[j] -
+                       fma(wx[j][2], syv[j][2], fma(wx[j][1], syv[j][1], wx[j][0] * syv[j][0])));
+    // z move: TSC along x and y at (x', y', q_z)
+    T wy[PPT][3];
+    int cy[PPT];
+    int ylo = 1 << 30, yhi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        tsc_axis_near((T)y[j] * Nf / Lf, cy[j], wy[j]);
+        ylo = cy[j] < ylo ? cy[j] : ylo;
+        yhi = cy[j] > yhi ? cy[j] : yhi;
+    }
+    wave_minmax(ylo, yhi);
+    const int ny = yhi - ylo + 3;
+    T szv[PPT][9];
+    if (stage_x && ny <= MAXY) {
+        // rows (x lo - 1 + i, y ylo - 1 + k) at buf[i * MAXY + k]; the y
+        // move's reads of buf precede these writes in the wave's order
+        unsigned oyk[MAXY];
+#pragma unroll
+        for (int k = 0; k < MAXY; k++) oyk[k] = (unsigned)wrapf(ylo - 1 + k, N, invN) * rs + zo;
+        for (int i = 0; i < nx; i += 2) {
+            const unsigned ox0 = (unsigned)wrapf(lo - 1 + i, N, invN) * plane;
+            const unsigned ox1 = (unsigned)wrapf(lo + i, N, invN) * plane;
+            T v[2 * MAXY];
+#pragma unroll
+            for (int k = 0; k < MAXY; k++) {
+                if (k < ny) v[k] = sz[ox0 + oyk[k]];
+                if (k < ny && i + 1 < nx) v[MAXY + k] = sz[ox1 + oyk[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < MAXY; k++) {
+                if (k < ny) buf[i * MAXY + k][lane] = v[k];
+                if (k < ny && i + 1 < nx) buf[(i + 1) * MAXY + k][lane] = v[MAXY + k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPT; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    szv[j][a * 3 + b] = buf[(cx[j] - lo + a) * MAXY + (cy[j] - ylo + b)][lane];
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            unsigned ox[3], oy[3];
+            tap_rows(wrapf(cx[j], N, invN), N, plane, ox);
+            tap_rows(wrapf(cy[j], N, invN), N, rs, oy);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) szv[j][a * 3 + b] = sz[ox[a] + oy[b] + zo];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        T d = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T r = fma(wy[j][2], szv[j][a * 3 + 2],
+                            fma(wy[j][1], szv[j][a * 3 + 1], wy[j][0] * szv[j][a * 3]));
+            d = fma(wx[j][a], r, d);
+        }
+        z[j] = (float)((T)z[j] - d);
+        const long long p = ((long long)qx[j] * N + qy) * N + qz;
+        const float Lp = (float)Lf;
+        X[p] = fwrap_float(x[j], Lp, boxlen);
+        Y[p] = fwrap_float(y[j], Lp, boxlen);
+        Z[p] = fwrap_float(z[j], Lp, boxlen);
     }
 }
 
@@ -446,17 +628,6 @@ __device__ __forceinline__ void cic_weights(double dx, double dy, double dz,
     w[5] = (unsigned long long)(dx * ty * dz * S + 0.5);
     w[6] = (unsigned long long)(tx * dy * dz * S + 0.5);
     w[7] = (unsigned long long)(dx * dy * dz * S + 0.5);
-}
-
-__device__ __noinline__ inline float fwrap_float_far(float x, double L) {
-    return (float)fwrap_fast(x, L);
-}
-
-// Float periodic wrap of a float position: x itself on [0, Lf), Lf = (float)L
-// (positions written by the displacement kernels lie there), the double wrap
-// otherwise.
-__device__ __forceinline__ float fwrap_float(float x, float Lf, double L) {
-    return (x >= 0.0f && x < Lf) ? x : fwrap_float_far(x, L);
 }
 
 // Float variant of cic_cell/cic_weights (FP32 pipeline): the offset in the

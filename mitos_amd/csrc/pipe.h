@@ -321,6 +321,9 @@ struct Pipe {
     // workspace of the two-pass lattice deposit (float weights, N >= 128);
     // null: the deposit adds its windows with atomics
     void* cic_ws = nullptr;
+    // the FP32 displacement stages its taps in LDS (MITOS_TSC_DIRECT=1: reads
+    // them from the grids, the same bits; for tests)
+    bool tsc_staged = true;
 
     // time every schedule variant of the plain passes (or reuse the choice
     // made for this device, N and precision)
@@ -452,6 +455,8 @@ struct Pipe {
         MITOS_HIP_CHECK(hipMalloc(&Z, sizeof(float) * np));
         typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
         const size_t wsb = cic_stream_bytes<M>(N);
+        const char* direct_env = getenv("MITOS_TSC_DIRECT");
+        tsc_staged = !(direct_env && direct_env[0] == '1');
         const char* atomic_env = getenv("MITOS_CIC_ATOMIC");
         const bool force_atomic = atomic_env && atomic_env[0] == '1';
         if (wsb && !force_atomic && hipMalloc(&cic_ws, wsb) != hipSuccess) {
@@ -684,7 +689,13 @@ struct Pipe {
                     const int nch = (NN + NT - 1) / NT;
                     const dim3 grid((unsigned)((long)NN * NN * nch));
                     const unsigned rs = (unsigned)(NN + MITOS_PSI_PAD);
-                    if (STENCIL) {
+                    if (STENCIL && NN % 64 == 0) {
+                        constexpr int TX = 16, PPT = 8;
+                        const dim3 gridw((unsigned)((long)NN * NN / PPT * (NN / 64)));
+                        hipLaunchKernelGGL((k_displace_lattice_w<T, TX, PPT, 16, 6>), gridw,
+                                           dim3(64), 0, s, psi(0), psi(1), psi(2), rs, NN,
+                                           (T)L, spacing, L, X, Y, Z, tsc_staged ? 1 : 0);
+                    } else if (STENCIL) {
                         constexpr int TX = NN % 16 == 0 ? 16 : 8;
                         constexpr int PPT = 4;
                         const dim3 gridp((unsigned)((long)NN * NN / PPT * nch));

@@ -140,3 +140,23 @@ def test_two_pass_deposit_matches_atomic(gpu, N, L, sigma8, monkeypatch):
     ok = a["obs"] > 0
     assert np.array_equal(a["P"][ok], b["P"][ok])
     assert abs(float(da.astype(np.float64).mean())) < 1e-6
+
+
+@pytest.mark.parametrize("N,L,sigma8", [(128, 250.0, 0.81), (128, 64.0, 3.0)])
+def test_staged_displacement_matches_direct(gpu, N, L, sigma8, monkeypatch):
+    # the FP32 displacement with its taps staged in LDS and with every tap
+    # read from the grids give the same positions, bit for bit (the second
+    # case spreads the taps beyond the staging window, so both paths run)
+    print("case staged displacement", N, L, sigma8, flush=True)
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=4, bins=16, sigma8=sigma8, precision=gpu.MITOS_FP32)
+    ic = gpu.IC(cfg)
+    ic.run()
+    pa = [t.cpu().numpy() for t in ic.positions()]
+    ic.close()
+    monkeypatch.setenv("MITOS_TSC_DIRECT", "1")
+    ic = gpu.IC(cfg)
+    ic.run()
+    pb = [t.cpu().numpy() for t in ic.positions()]
+    ic.close()
+    for a, b in zip(pa, pb):
+        assert np.array_equal(a, b)
