@@ -128,24 +128,48 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// fold the partials of nblk work-groups, in order; out[b*2 + {P_sum, K_sum}],
-// obs[b] (the k sums and counts from the geometry tables)
-static __global__ void k_pk_final(const double* part, int nblk, int bins,
-                                  const double* __restrict__ ksum,
-                                  const long long* __restrict__ kobs, double* out,
-                                  long long* obs) {
+// fold the partials of nblk work-groups in a fixed order: one wave per bin,
+// lane l folds work-groups l, l + 64, ... in order, then lane 0 folds the 64
+// lane sums in lane order (compensated throughout); out[b*2 + {P_sum,
+// K_sum}], obs[b] (the k sums and counts from the geometry tables)
+static __global__ void __launch_bounds__(64)
+    k_pk_final(const double* part, int nblk, int bins, const double* __restrict__ ksum,
+               const long long* __restrict__ kobs, double* out, long long* obs) {
 #pragma clang fp contract(off)
-    const int bn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (bn >= bins) return;
+    __shared__ double ls[64][2];
+    const int bn = blockIdx.x;
+    const int lane = threadIdx.x;
     double sP = 0, cP = 0;
-    for (int g = 0; g < nblk; g++) {
+    int g = lane;
+    for (; g + 3 * 64 < nblk; g += 4 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double* o = part + ((size_t)(g + u * 64) * bins + bn) * 2;
+            v[2 * u] = o[0];
+            v[2 * u + 1] = o[1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) kadd(sP, cP, v[u]);
+    }
+    for (; g < nblk; g += 64) {
         const double* o = part + ((size_t)g * bins + bn) * 2;
         kadd(sP, cP, o[0]);
         kadd(sP, cP, o[1]);
     }
-    out[bn * 2 + 0] = sP + cP;
-    out[bn * 2 + 1] = ksum[bn];
-    obs[bn] = kobs[bn];
+    ls[lane][0] = sP;
+    ls[lane][1] = cP;
+    __syncthreads();
+    if (lane == 0) {
+        double s = 0, c = 0;
+        for (int l = 0; l < 64; l++) {
+            kadd(s, c, ls[l][0]);
+            kadd(s, c, ls[l][1]);
+        }
+        out[bn * 2 + 0] = s + c;
+        out[bn * 2 + 1] = ksum[bn];
+        obs[bn] = kobs[bn];
+    }
 }
 
 // Host: per-bin mode counts (with the multiplicity of src/calc_powerspec.c:
@@ -168,7 +192,7 @@ inline float pk_bin_cinv(int N, int bins) {
 // Host: number of work-groups used by k_pk_partial
 inline int pk_blocks(long nmodes) {
     long b = (nmodes + 255) / 256;
-    return (int)(b < 2048 ? b : 2048);
+    return (int)(b < 8192 ? b : 8192);
 }
 
 // Binned P(k) of two half-complex grids with row stride hs; host outputs
@@ -200,7 +224,7 @@ inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen,
     hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, edges,
                        pk_bin_cinv(N, bins), bins, part, xf);
     MITOS_TRY(launch_status());
-    hipLaunchKernelGGL(k_pk_final, dim3((bins + 63) / 64), dim3(64), 0, s, part, nblk, bins,
+    hipLaunchKernelGGL(k_pk_final, dim3(bins), dim3(64), 0, s, part, nblk, bins,
                        geo.ksum, geo.obs, fs, fo);
     MITOS_TRY(launch_status());
     if (sums) {
