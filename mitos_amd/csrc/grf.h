@@ -68,8 +68,8 @@ __host__ __device__ __forceinline__ void wavevec(int x, int y, int z, int N, dou
     k = sqrt(kx * kx + ky * ky + kz * kz);
 }
 
-// src/grf.c:26-69 with an amplitude amp(n2, kx, ky, kz, k) applied to both
-// parts (n2 = integer |k|^2 / dk^2).  One thread walks one z-row in stream
+// src/grf.c:26-69 with an amplitude amp(n2) applied to both parts (n2 =
+// integer |k|^2 / dk^2).  One thread walks one z-row in stream
 // order; a block's 256 rows are staged through LDS in chunks of 16 modes so
 // the stores are 16-mode contiguous runs.
 template <typename T, typename AMP, bool CR>
@@ -86,7 +86,6 @@ __global__ void __launch_bounds__(256)
     Xo gen = ok ? row_start(J, line) : J.s0;
     const double boxvol = boxlen * boxlen * boxlen;
     const double factor = sqrt(boxvol / 2);
-    const double dk = 2 * M_PI / boxlen;
     const int H = N / 2 + 1;
     const int nx = freq_index(x, N), ny = freq_index(y, N);
     const long n2xy = (long)nx * nx + (long)ny * ny;
@@ -96,10 +95,10 @@ __global__ void __launch_bounds__(256)
             for (int dz = 0; dz < ZC; dz++) {
                 const int z = z0 + dz;
                 if (z >= H) break;
-                double kx, ky, kz, k;
-                wavevec(x, y, z, N, dk, kx, ky, kz, k);
+                // the DC mode (n^2 = 0, k = 0) takes no draws
+                const long n2 = n2xy + (long)z * z;
                 cx<T> val = mk<T>(0, 0);
-                if (k > 0) {
+                if (n2 > 0) {
                     double a, b;
                     if constexpr (sizeof(T) == 4) {
                         a = (double)bm_normal_f(gen) * factor;
@@ -108,7 +107,7 @@ __global__ void __launch_bounds__(256)
                         a = bm_normal<CR>(gen) * factor;
                         b = bm_normal<CR>(gen) * factor;
                     }
-                    const double m = amp(n2xy + (long)z * z, kx, ky, kz, k);
+                    const double m = amp(n2);
                     val = mk<T>((T)(a * m), (T)(b * m));
                 }
                 stage[threadIdx.x * ZC + dz] = val;
@@ -150,17 +149,13 @@ __global__ void k_hermitian(cx<T>* f, long hs, int N, long batch_stride) {
 }
 
 struct AmpOne {
-    __device__ __forceinline__ double operator()(long, double, double, double, double) const {
-        return 1.0;
-    }
+    __device__ __forceinline__ double operator()(long) const { return 1.0; }
 };
 
 // amplitude looked up by the integer n^2 of the mode
 struct AmpTable {
     const double* tab;
-    __device__ __forceinline__ double operator()(long n2, double, double, double, double) const {
-        return tab[n2];
-    }
+    __device__ __forceinline__ double operator()(long n2) const { return tab[n2]; }
 };
 
 template <typename T, typename AMP, bool CR = true>

@@ -224,7 +224,9 @@ struct HessZK<T, N, Radices<Rs...>> {
         constexpr int L = N / 2;
         constexpr int E = LineCfgS<L>::E;
         constexpr int TPL = L / E;
-        constexpr int RW = RowsPerBlock<TPL>::value;
+        // a row of a wave or more: one row per work-group (no barrier waits
+        // on other rows' waves)
+        constexpr int RW = TPL >= 64 ? 1 : RowsPerBlock<TPL>::value;
         const long nrows = (long)N * N;
         const cx<T>* tw = twiddles<T>(L);
         const cx<T>* twN = twiddles<T>(N);
