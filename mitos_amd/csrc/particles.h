@@ -836,14 +836,24 @@ __device__ __forceinline__ void cic_global_near(unsigned long long* grid, int N,
 }
 
 // Float cell of one coordinate of a position in [0, L] (the displacement
-// kernels write positions wrapped into the box): i = floor(x/cell) clamped
-// to [0, N] (N is cell 0 after the periodic offset below; the clamp keeps
-// any input inside the grid), d = x/cell - i with one rounding.
-__device__ __forceinline__ void cic_axis_f(float x, float cinv, int N, int& i, float& d) {
-    int k = (int)(x * cinv);
-    k = k < 0 ? 0 : (k > N ? N : k);
-    i = k;
-    d = fmaf(x, cinv, -(float)k);
+// kernels write positions wrapped into the box): the offset in the cell is
+// formed as x - i * cell with one rounding (cell = L/N is exact for the usual
+// box and grid sizes, the reciprocal is not), then scaled to [0, 1]; i is
+// clamped to [0, N] (N is cell 0 after the periodic offset below; the clamp
+// keeps any input inside the grid).
+__device__ __forceinline__ void cic_axis_f(float x, float cell, float cinv, int N, int& i,
+                                           float& d) {
+    int k = (int)floorf(x * cinv);
+    float r = fmaf(-(float)k, cell, x);
+    if (r < 0.0f) {
+        k--;
+        r += cell;
+    } else if (r >= cell) {
+        k++;
+        r -= cell;
+    }
+    i = k < 0 ? 0 : (k > N ? N : k);
+    d = fminf(r * cinv, 1.0f);
 }
 
 // Cell of a lattice particle at site (qx, qy, qz) with position (x, y, z)
@@ -859,10 +869,9 @@ __device__ __forceinline__ void cic_lattice_cell(float x, float y, float z, int 
     if constexpr (sizeof(W) == 8) {
         cic_cell(x, y, z, N, boxlen, iX, iY, iZ, fx, fy, fz);
     } else {
-        (void)cell;
-        cic_axis_f(x, cinv, N, iX, fx);
-        cic_axis_f(y, cinv, N, iY, fy);
-        cic_axis_f(z, cinv, N, iZ, fz);
+        cic_axis_f(x, cell, cinv, N, iX, fx);
+        cic_axis_f(y, cell, cinv, N, iY, fy);
+        cic_axis_f(z, cell, cinv, N, iZ, fz);
     }
     rx = iX - qx;
     ry = iY - qy;
