@@ -8,6 +8,15 @@
  * that choice cannot be reproduced without its binary and is documented).
  *
  * Every function names the reference file:line it restates.
+ *
+ * Pins (tests/test_oracle_pins.py): the RNG against golden draws of the
+ * reference's own random_xorshift.h built from source; the FFTs against
+ * numpy and a brute-force DFT; the kernels, Poisson and 2LPT against closed
+ * forms; P(k) counts against an enumeration of the reference formula.
+ * Parity unpinned against the reference itself (no reference output can be
+ * produced here): the c2r of non-Hermitian Nyquist planes (FFTW leaves it
+ * unspecified; numpy's convention is used) and the host-libm rounding of
+ * log/cos in Box-Muller (pinned on this host's glibc only).
  */
 #include "mitos_oracle.h"
 
