@@ -312,7 +312,7 @@ int mitos_ic_destroy(mitos_ic_context *ctx);
 
 /* FFT schedule picked by the autotuner at mitos_ic_create for pass `kind`
  * (0: y forward, 1: x forward, 2: z c2r, 3: z r2c): returns the variant
- * (0..2) and fills ms[3] with the timed milliseconds of each variant (all 0
+ * (0..3) and fills ms[4] with the timed milliseconds of each variant (all 0
  * when MITOS_FFT_SCHEDULE=default disabled the timing); -1 on bad arguments */
 int mitos_ic_schedule(mitos_ic_context *ctx, int kind, float *ms);
 /* mean milliseconds of a plain 3D r2c and c2r of the context's size and

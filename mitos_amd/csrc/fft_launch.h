@@ -180,21 +180,41 @@ int zr2c_pass(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op, 
 //   strided  0: LineCfg radices, 128-byte column tile
 //            1: LineCfgS radices (fewer values per thread), the widest tile
 //               of at most 1024 threads
-//            2: LineCfg radices, 256-byte column tile when it fits
+//            2: 256-byte column tile when it fits (32 values per thread for
+//               1024-point float lines)
+//            3: LineCfg radices, 128-byte tile, two LDS exchange buffers
+//               (one exchange step fewer) when they fit in 160 KiB
 //   z passes 0: LineCfg radices, RowsPerBlock rows per work-group
 //            1: LineCfgS radices, RowsPerBlock rows of its thread count
 //            2: LineCfg radices, twice the rows when it fits
-constexpr int kFftVariants = 3;
+//            3: LineCfg radices, one row per work-group when a row fills a
+//               wave (no barrier waits on other rows)
+constexpr int kFftVariants = 4;
+
+// 32 values per thread for the 256-byte tile of 1024-point float lines
+template <int L>
+struct LineCfgW {
+    static constexpr int E = LineCfg<L>::E;
+    typedef typename LineCfg<L>::R R;
+};
+template <>
+struct LineCfgW<1024> {
+    static constexpr int E = 32;
+    typedef Radices<16, 16, 4> R;
+};
 
 template <typename T, int L, int V>
 struct StridedSched {
-    typedef typename std::conditional<V == 1, LineCfgS<L>, LineCfg<L>>::type CFG;
+    typedef typename std::conditional<
+        V == 1, LineCfgS<L>,
+        typename std::conditional<V == 2 && sizeof(T) == 4, LineCfgW<L>, LineCfg<L>>::type>::type
+        CFG;
     static constexpr int TPL = L / CFG::E;
     static constexpr int CB = (TPL * ColTile<T>::C <= 1024) ? ColTile<T>::C : 1024 / TPL;
-    // the wider tile only while one exchange buffer stays within 80 KiB of LDS
+    // the wider tile only while one exchange buffer fits in the LDS
     static constexpr int C2 = 2 * ColTile<T>::C;
     static constexpr bool WIDE = V == 2 && TPL * C2 <= 1024 &&
-                                 (long)L * (C2 + 1) * (long)sizeof(T) <= 81920;
+                                 (long)L * (C2 + 1) * (long)sizeof(T) <= 163840;
     static constexpr int C = WIDE ? C2 : CB;
 };
 
@@ -216,7 +236,8 @@ int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstrid
                    hipStream_t s) {
     typedef StridedSched<T, L, V> S;
     constexpr int C = S::C;
-    constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
+    constexpr long LDS2 = 2 * (long)LdsMap<T, L, C, true>::SIZE * (long)sizeof(T);
+    constexpr bool SPLIT = V == 3 ? LDS2 > 163840 : LDS2 > 81920;
     const int ncols = L / 2 + 1;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
@@ -233,6 +254,7 @@ int strided_pass_tuned(int variant, const cx<T>* in, cx<T>* out, long hs, int nb
     switch (variant) {
         case 1: return strided_pass_v<T, L, AXIS, DIR, 1>(in, out, hs, nbatch, bstride, op, s);
         case 2: return strided_pass_v<T, L, AXIS, DIR, 2>(in, out, hs, nbatch, bstride, op, s);
+        case 3: return strided_pass_v<T, L, AXIS, DIR, 3>(in, out, hs, nbatch, bstride, op, s);
         default: return strided_pass_v<T, L, AXIS, DIR, 0>(in, out, hs, nbatch, bstride, op, s);
     }
 }
@@ -241,7 +263,8 @@ template <typename T, int L, int V>
 struct ZSched {
     typedef typename std::conditional<V == 1, LineCfgS<L>, LineCfg<L>>::type CFG;
     static constexpr int TPL = L / CFG::E;
-    static constexpr int RW0 = RowsPerBlock<TPL>::value * (V == 2 ? 2 : 1);
+    static constexpr int RW0 = V == 3 ? (TPL >= 64 ? 1 : RowsPerBlock<TPL>::value)
+                                      : RowsPerBlock<TPL>::value * (V == 2 ? 2 : 1);
     static constexpr int RW = (TPL * RW0 <= 1024) ? RW0 : RowsPerBlock<TPL>::value;
 };
 
@@ -279,6 +302,7 @@ int zc2r_pass_tuned(int variant, const cx<T>* in, T* out, long hs, long rs, int 
     switch (variant) {
         case 1: return zc2r_pass_v<T, N, 1, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
         case 2: return zc2r_pass_v<T, N, 2, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
+        case 3: return zc2r_pass_v<T, N, 3, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
         default: return zc2r_pass_v<T, N, 0, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
     }
 }
@@ -315,6 +339,7 @@ int zr2c_pass_tuned(int variant, LOADER loader, cx<T>* out, long hs, int nbatch,
     switch (variant) {
         case 1: return zr2c_pass_v<T, N, 1>(loader, out, hs, nbatch, outb, op, s);
         case 2: return zr2c_pass_v<T, N, 2>(loader, out, hs, nbatch, outb, op, s);
+        case 3: return zr2c_pass_v<T, N, 3>(loader, out, hs, nbatch, outb, op, s);
         default: return zr2c_pass_v<T, N, 0>(loader, out, hs, nbatch, outb, op, s);
     }
 }

@@ -359,11 +359,14 @@ struct Pipe {
                 };
                 CicLoader<T> ld{cic(), (long)NN, 1.0};
                 for (int v = 0; v < kFftVariants; v++) {
+                    // out of place, as the runs use them
                     MITOS_TRY(time_it([&] {
-                        return strided_pass_tuned<T, NN, 1, -1>(v, K, K, hs, 1, 0, OpNone(), s);
+                        return strided_pass_tuned<T, NN, 1, -1>(v, K, a(0), hs, 1, 0, OpNone(),
+                                                                s);
                     }, &sc.ms[TUNE_Y_FWD][v]));
                     MITOS_TRY(time_it([&] {
-                        return strided_pass_tuned<T, NN, 0, -1>(v, K, K, hs, 1, 0, OpNone(), s);
+                        return strided_pass_tuned<T, NN, 0, -1>(v, a(0), K, hs, 1, 0, OpNone(),
+                                                                s);
                     }, &sc.ms[TUNE_X_FWD][v]));
                     MITOS_TRY(time_it([&] {
                         return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(
@@ -402,15 +405,16 @@ struct Pipe {
             auto r2c = [&]() -> int {
                 MITOS_TRY((zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, K, hs, 1, 0, OpNone(),
                                                   s)));
-                MITOS_TRY((strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, K, hs, 1, 0,
-                                                            OpNone(), s)));
-                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], K, K, hs, 1, 0,
+                // out of place (an in-place strided pass streams slower)
+                MITOS_TRY((strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, a(0), hs, 1,
+                                                            0, OpNone(), s)));
+                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], a(0), K, hs, 1, 0,
                                                         OpNone(), s);
             };
             auto c2r = [&]() -> int {
                 MITOS_TRY((strided_pass<T, NN, 0, 1>(K, a(0), hs, 1, 0, OpNone(), s)));
-                MITOS_TRY((strided_pass<T, NN, 1, 1>(a(0), a(0), hs, 1, 0, OpNone(), s)));
-                return zc2r_pass_tuned<T, NN, 0>(sched.v[TUNE_Z_C2R], a(0), real, hs, rs, 1, 0, 0,
+                MITOS_TRY((strided_pass<T, NN, 1, 1>(a(0), a(1), hs, 1, 0, OpNone(), s)));
+                return zc2r_pass_tuned<T, NN, 0>(sched.v[TUNE_Z_C2R], a(1), real, hs, rs, 1, 0, 0,
                                                  (T)1, OpNone(), s);
             };
             MITOS_TRY(r2c());
@@ -594,11 +598,12 @@ struct Pipe {
                     return zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, D, hs, 1, 0, OpNone(),
                                                   s);
                 }));
-                // 5. y pass forward of delta2
+                // 5. y pass forward of delta2, out of place into a(2)
                 MITOS_TRY(timed(s, "delta2_y_fwd", 2 * G, [&] {
-                    return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], D, D, hs, 1, 0,
-                                                            OpNone(), s);
+                    return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], D, a(2), hs, 1,
+                                                            0, OpNone(), s);
                 }));
+                D = a(2);
             } else {
                 D = K;  // Zel'dovich: the second-order term is multiplied by zero
                 mark(2);
@@ -619,13 +624,14 @@ struct Pipe {
                 cp.smooth = smooth_table<double>(NN);
                 if (!cp.smooth) return MITOS_ERR_ALLOC;
             }
+            // C0, C1 into a(0), a(1) (D is a(2) or K)
             MITOS_TRY(timed(s, "combine_x", 4 * G, [&] {
-                return CombineK<T, NN>::go(D, K, a(1), a(2), hs, cp, s);
+                return CombineK<T, NN>::go(D, K, a(0), a(1), hs, cp, s);
             }));
             // 7. y pass: E1 = Y(i ky C0) (FP32: i ky S(ky) C0), E2 = Y(C0), E0 = Y(C1)
             MultiPtrs<T, 2, 3> p7;
-            p7.in[0] = a(1);
-            p7.in[1] = a(2);
+            p7.in[0] = a(0);
+            p7.in[1] = a(1);
             p7.out[0] = b(1);
             p7.out[1] = b(2);
             p7.out[2] = b(0);
@@ -749,12 +755,14 @@ struct Pipe {
             MITOS_TRY(timed(s, "density_z_r2c", 8.0 * np + G, [&] {
                 return zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, K, hs, 1, 0, OpNone(), s);
             }));
+            // y and x out of place through a(0) (free once the displacement
+            // grids are made)
             MITOS_TRY(timed(s, "density_y_fwd", 2 * G, [&] {
-                return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, K, hs, 1, 0,
+                return strided_pass_tuned<T, NN, 1, -1>(sched.v[TUNE_Y_FWD], K, a(0), hs, 1, 0,
                                                         OpNone(), s);
             }));
             MITOS_TRY(timed(s, "density_x_fwd", 2 * G, [&] {
-                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], K, K, hs, 1, 0,
+                return strided_pass_tuned<T, NN, 0, -1>(sched.v[TUNE_X_FWD], a(0), K, hs, 1, 0,
                                                         OpNone(), s);
             }));
             const double* w2 = window_table(NN, L, 2);

@@ -269,7 +269,7 @@ class IC:
         """Autotuned FFT schedule per pass kind: (variant, [ms per variant])."""
         out = {}
         for kind, name in enumerate(("y_fwd", "x_fwd", "z_c2r", "z_r2c")):
-            ms = (ctypes.c_float * 3)()
+            ms = (ctypes.c_float * 4)()
             v = lib().mitos_ic_schedule(self.h, kind, ms)
             out[name] = (v, [float(x) for x in ms])
         return out

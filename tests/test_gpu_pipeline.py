@@ -96,7 +96,7 @@ def test_fused_repeatable(gpu):
     ic.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_every_fft_schedule_variant(gpu, oracle, variant):
     # each schedule the autotuner can pick gives the oracle's P(k) and psi
     N, L, seed, bins = 32, 128.0, 11, 16
@@ -176,3 +176,26 @@ def test_fp32_pk_vs_fp64_256(gpu):
     err = _pk_rel(a["P"], b["P"], b["obs"])
     print("  max", err.max(), "rms", np.sqrt((err ** 2).mean()), flush=True)
     assert err.max() < 1e-5
+
+
+def test_every_fft_schedule_variant_128(gpu, oracle):
+    # at 128^3 the variants differ in tile widths and rows per work-group as
+    # at the large sizes; each gives the oracle's P(k)
+    N, L, seed, bins = 128, 250.0, 13, 24
+    ref = _oracle_run(oracle, N, L, seed, bins)
+    ok = ref["obs"] > 1
+    for variant in range(4):
+        print("case schedule variant 128", variant, flush=True)
+        for prec in (gpu.MITOS_FP64, gpu.MITOS_FP32):
+            gpu.set_fft_schedule(N, prec, [variant] * 4)
+            ic = gpu.IC(gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=prec))
+            P = ic.run()["P"]
+            ic.close()
+            err = _pk_rel(P, ref["P"], ref["obs"])
+            nrel = np.linalg.norm(P[ok] - ref["P"][ok]) / np.linalg.norm(ref["P"][ok])
+            if prec == gpu.MITOS_FP64:
+                # float positions: a last-bit difference of psi moves a few
+                # particles by one float ulp (the FP64 contract is norm-relative)
+                assert nrel < 1e-10 and err.max() < 1e-6
+            else:
+                assert err.max() < 1e-4
