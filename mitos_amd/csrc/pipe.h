@@ -187,9 +187,13 @@ inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); 
 // Line configuration and tile width of the register-heavy multi-transform
 // kernels: 8 (or 12) values per thread and 64-byte tile rows, so that the
 // input tile, the transformed copy and the radix temporaries stay in VGPRs.
-template <typename T, int L>
+// Multi-output passes on float lines of 1024: 16 values per thread, so a
+// line is one wave (fewer exchange steps, work-groups of 512 threads); the
+// combine pass keeps the smaller line configuration (MULTI = false).
+template <typename T, int L, bool MULTI = true>
 struct HeavyCfg {
-    typedef LineCfgS<L> CFG;
+    typedef typename std::conditional<MULTI && sizeof(T) == 4 && L == 1024, LineCfg<L>,
+                                      LineCfgS<L>>::type CFG;
     static constexpr int E = CFG::E;
     static constexpr int TPL = L / E;
     static constexpr int C0 = 64 / (2 * (int)sizeof(T));
@@ -237,15 +241,15 @@ struct HessZK<T, N, Radices<Rs...>> {
     }
 };
 
-template <typename T, int N, typename RAD = typename HeavyCfg<T, N>::CFG::R>
+template <typename T, int N, typename RAD = typename HeavyCfg<T, N, false>::CFG::R>
 struct CombineK;
 template <typename T, int N, int... Rs>
 struct CombineK<T, N, Radices<Rs...>> {
     static int go(const cx<T>* D, const cx<T>* K, cx<T>* C0, cx<T>* C1, long hs,
                   const CombineParams& cp, hipStream_t s) {
         constexpr int L = N;
-        constexpr int C = HeavyCfg<T, L>::C;
-        constexpr int E = HeavyCfg<T, L>::E;
+        constexpr int C = HeavyCfg<T, L, false>::C;
+        constexpr int E = HeavyCfg<T, L, false>::E;
         constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
         const int ncols = L / 2 + 1;
         const cx<T>* tw = twiddles<T>(L);
