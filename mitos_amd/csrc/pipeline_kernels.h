@@ -118,22 +118,33 @@ __global__ void __launch_bounds__(L / E * RW)
     // xx -> A; yy -> B = xx + yy, A = xx * yy; zz -> A = zz * B + A; then
     // A -= h^2 for xy, yz, xz: two accumulator arrays besides the transform
     cx<T> A[E], B[E], cur[E];
-    // component order: xx, yy, zz, xy, yz, xz
+    // component order: xx, yy, zz, xy, yz, xz; the next component's row is
+    // loaded while the current one is transformed
+    auto src_of = [](int q) {
+        return (q == 0) ? 5 : (q == 1) ? 2 : (q == 2) ? 0 : (q == 3) ? 4 : (q == 4) ? 1 : 3;
+    };
+    cx<T> nxt[E + 1];
+    auto load_row = [&](int q) {
+        const cx<T>* src = P.in[src_of(q)] + row * hs;
+#pragma unroll
+        for (int e = 0; e < E; e++) nxt[e] = ld(src + t + TPL * e);
+        nxt[E] = t == 0 ? ld(src + L) : mk<T>(0, 0);
+    };
+    if (ok) load_row(0);
 #pragma unroll 1
     for (int q = 0; q < 6; q++) {
-        const int src_q = (q == 0) ? 5 : (q == 1) ? 2 : (q == 2) ? 0 : (q == 3) ? 4 : (q == 4) ? 1 : 3;
         const int mul_q = (q == 2) ? 2 : (q >= 4) ? 1 : 0;
         // stage the row, with its k_z multiplier, in LDS
         if (ok) {
-            const cx<T>* src = P.in[src_q] + row * hs;
 #pragma unroll
             for (int e = 0; e <= E; e++) {
                 const int k = (e < E) ? t + TPL * e : L;
                 if (e == E && t != 0) break;
-                cx<T> a = apply_mul(mul_q, ld(src + k), (T)(k * dk));
+                cx<T> a = apply_mul(mul_q, nxt[e], (T)(k * dk));
                 lre[M::idx(k, rl)] = a.re;
                 lim[M::idx(k, rl)] = a.im;
             }
+            if (q + 1 < 6) load_row(q + 1);
         }
         __syncthreads();
 #pragma unroll
