@@ -110,9 +110,12 @@ extern "C" int alloc_local_grid(struct distributed_grid* dg, int N, double boxle
         hipFree(f);
         return set_last_error(MITOS_ERR_ALLOC);
     }
-    // zero-filled, so padding entries are defined
-    if (hipMemset(f, 0, sizeof(cx<double>) * local) != hipSuccess ||
-        hipMemset(b, 0, sizeof(double) * 2 * local) != hipSuccess) {
+    // zero-filled, so padding entries are defined; on the library stream and
+    // finished before return (the kernels of later calls run on that
+    // non-blocking stream, which a null-stream memset would not order)
+    if (hipMemsetAsync(f, 0, sizeof(cx<double>) * local, api_stream()) != hipSuccess ||
+        hipMemsetAsync(b, 0, sizeof(double) * 2 * local, api_stream()) != hipSuccess ||
+        api_sync() != MITOS_OK) {
         hipFree(f);
         hipFree(b);
         return set_last_error(MITOS_ERR_DEVICE);
