@@ -337,7 +337,7 @@ static int backward(Pipe<T>* P, const mitos_ic_config& cfg, const cx<T>* noise,
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(n3)), dim3(256), 0, s, Ay, n3, (T)0);
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(n3)), dim3(256), 0, s, Az, n3, (T)0);
     hipLaunchKernelGGL((k_adj_particles<T>), dim3(nblk(n3)), dim3(256), 0, s, P->psi(0),
-                       P->psi(1), P->psi(2), (unsigned)(N + MITOS_PSI_PAD), grho, Ax, Ay, Az, N,
+                       P->psi(1), P->psi(2), (unsigned)P->prs(), grho, Ax, Ay, Az, N,
                        L, (float)L / N);
     MITOS_TRY(launch_status());
     // 3. back to k-space: R_a = r2c(A_a)

@@ -259,6 +259,15 @@ int strided_pass_tuned(int variant, const cx<T>* in, cx<T>* out, long hs, int nb
     }
 }
 
+// work-groups of a z pass: each walks several row groups (about 8192 waves
+// in all, enough to fill the device), at most one per group
+inline unsigned z_groups(long nrows, int rw, int tpl) {
+    const long ngroups = (nrows + rw - 1) / rw;
+    const long waves = ((long)tpl * rw + 63) / 64;
+    const long cap = 8192 / waves > 0 ? 8192 / waves : 1;
+    return (unsigned)(ngroups < cap ? ngroups : cap);
+}
+
 template <typename T, int L, int V>
 struct ZSched {
     typedef typename std::conditional<V == 1, LineCfgS<L>, LineCfg<L>>::type CFG;
@@ -289,7 +298,7 @@ int zc2r_pass_v(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb,
     const cx<T>* tw = twiddles<T>(L);
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
-    dim3 grid((unsigned)((nrows + S::RW - 1) / S::RW), nbatch);
+    dim3 grid(z_groups(nrows, S::RW, S::TPL), nbatch);
     ZC2RKV<T, L, S::RW, HO, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, in, out,
                                                      nrows, N, hs, rs, inb, outb, tw, twN, scale,
                                                      op);
@@ -327,7 +336,7 @@ int zr2c_pass_v(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op
     const cx<T>* tw = twiddles<T>(L);
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
-    dim3 grid((unsigned)((nrows + S::RW - 1) / S::RW), nbatch);
+    dim3 grid(z_groups(nrows, S::RW, S::TPL), nbatch);
     ZR2CKV<T, L, S::RW, LOADER, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, loader,
                                                          out, nrows, N, hs, outb, tw, twN, op);
     return launch_status();

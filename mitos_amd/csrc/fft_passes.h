@@ -76,6 +76,16 @@ struct OpScale {
     __device__ __forceinline__ cx<T> post(cx<T> v, int, int, int) const { return cscale(v, s); }
 };
 
+// Columns a strided pass stores: the valid ones rounded up to whole tiles of C
+// while the row padding allows.  The last tile then writes whole 128-byte lines
+// (zeros in the padding columns) instead of one partial line per row, which the
+// memory system completes with an extra read.
+template <int C>
+__device__ __forceinline__ int stored_cols(int ncols, long hs) {
+    const long r = (long)(ncols + C - 1) / C * C;
+    return (int)(r < hs ? r : hs);
+}
+
 // ---- strided c2c pass ------------------------------------------------------
 // AXIS 0: lines along x, blockIdx.y = y;  AXIS 1: lines along y, blockIdx.y = x.
 // blockIdx.z selects one of several grids spaced `batch_stride` apart.
@@ -95,6 +105,7 @@ __global__ void __launch_bounds__(L / E * C)
     const int col = bx * C + c;
     const int o = by;
     const bool ok = col < ncols;
+    const bool wok = col < stored_cols<C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = blockIdx.z * batch_stride + (long)o * ostride + col;
@@ -106,11 +117,12 @@ __global__ void __launch_bounds__(L / E * C)
         v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
     }
     line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-    if (ok) {
+    if (wok) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int p = t + TPL * e;
-            cx<T> a = (AXIS == 0) ? op.post(v[e], p, o, col) : op.post(v[e], o, p, col);
+            cx<T> a = !ok ? mk<T>(0, 0)
+                          : (AXIS == 0) ? op.post(v[e], p, o, col) : op.post(v[e], o, p, col);
             st(out + base + (long)p * pstride, a);
         }
     }
@@ -121,6 +133,8 @@ __global__ void __launch_bounds__(L / E * C)
 // the N values at row[0..N), the remaining rs-N set to zero.  HO > 0: value z
 // at row[HO + z] with the periodic halo row[HO-1] = value N-1 and row[HO+N] =
 // value 0 (rs >= N + HO + 1; the other padding is left unwritten).  L = N/2.
+// Each work-group walks row groups blockIdx.x, blockIdx.x + gridDim.x, ...
+// and loads the next group's row while transforming the current one.
 template <typename T, int L, int E, int RW, int HO, typename OP, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
     k_zc2r(const cx<T>* __restrict__ in, T* __restrict__ out, long nrows, int N, long hs,
@@ -132,60 +146,82 @@ __global__ void __launch_bounds__(L / E * RW)
     __shared__ T lim[M::SIZE];
     const int t = threadIdx.x % TPL;
     const int rl = threadIdx.x / TPL;
-    const long row = (long)blockIdx.x * RW + rl;
-    const bool ok = row < nrows;
-    const cx<T>* src = in + blockIdx.y * in_batch + row * hs;
-    const int gx = (int)(row / N), gy = (int)(row % N);
-    // stage the row (positions 0..L) in LDS
-    if (ok) {
+    const long ngroups = (nrows + RW - 1) / RW;
+    cx<T> nxt[E + 1];
+    auto load = [&](long g) {
+        const long row = g * RW + rl;
+        if (row < nrows) {
+            const cx<T>* src = in + blockIdx.y * in_batch + row * hs;
+#pragma unroll
+            for (int e = 0; e < E; e++) nxt[e] = ld(src + t + TPL * e);
+            nxt[E] = t == 0 ? ld(src + L) : mk<T>(0, 0);
+        }
+    };
+    if ((long)blockIdx.x < ngroups) load(blockIdx.x);
+    for (long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const long row = g * RW + rl;
+        const bool ok = row < nrows;
+        const int gx = (int)(row / N), gy = (int)(row % N);
+        // stage the row (positions 0..L) in LDS
+        if (ok) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + TPL * e;
+                cx<T> a = op.pre(nxt[e], gx, gy, k);
+                lre[M::idx(k, rl)] = a.re;
+                lim[M::idx(k, rl)] = a.im;
+            }
+            if (t == 0) {
+                cx<T> a = op.pre(nxt[E], gx, gy, L);
+                lre[M::idx(L, rl)] = a.re;
+                lim[M::idx(L, rl)] = a.im;
+            }
+        }
+        if (g + gridDim.x < ngroups) load(g + gridDim.x);
+        __syncthreads();
+        cx<T> v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int k = t + TPL * e;
-            cx<T> a = op.pre(ld(src + k), gx, gy, k);
-            lre[M::idx(k, rl)] = a.re;
-            lim[M::idx(k, rl)] = a.im;
+            cx<T> xk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+            cx<T> xm = mk<T>(lre[M::idx(L - k, rl)], lim[M::idx(L - k, rl)]);
+            if (k == 0) {  // Im of the DC and Nyquist entries is ignored
+                xk.im = 0;
+                xm.im = 0;
+            }
+            cx<T> a = xk + conjg(xm);
+            cx<T> b = xk - conjg(xm);
+            cx<T> w = twN[k];
+            w.im = -w.im;  // exp(+2 pi i k / N)
+            v[e] = a + mul_i<1>(cmul(w, b));
         }
-        if (t == 0) {
-            cx<T> a = op.pre(ld(src + L), gx, gy, L);
-            lre[M::idx(L, rl)] = a.re;
-            lim[M::idx(L, rl)] = a.im;
-        }
-    }
-    __syncthreads();
-    cx<T> v[E];
+        __syncthreads();
+        line_fft<T, L, E, RW, false, 1, Rs...>(v, lre, lim, tw, t, rl);
+        if (ok) {
+            T* dst = out + blockIdx.y * out_batch + row * rs;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int k = t + TPL * e;
-        cx<T> xk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
-        cx<T> xm = mk<T>(lre[M::idx(L - k, rl)], lim[M::idx(L - k, rl)]);
-        if (k == 0) {  // Im of the DC and Nyquist entries is ignored
-            xk.im = 0;
-            xm.im = 0;
-        }
-        cx<T> a = xk + conjg(xm);
-        cx<T> b = xk - conjg(xm);
-        cx<T> w = twN[k];
-        w.im = -w.im;  // exp(+2 pi i k / N)
-        v[e] = a + mul_i<1>(cmul(w, b));
-    }
-    __syncthreads();
-    line_fft<T, L, E, RW, false, 1, Rs...>(v, lre, lim, tw, t, rl);
-    if (ok) {
-        T* dst = out + blockIdx.y * out_batch + row * rs;
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int m = t + TPL * e;
-            typename vec2<T>::type p;
-            p.x = v[e].re * scale;
-            p.y = v[e].im * scale;
-            *reinterpret_cast<typename vec2<T>::type*>(dst + HO + 2 * m) = p;
-            if (HO > 0) {
-                if (m == 0) dst[HO + 2 * L] = p.x;
-                if (m == L - 1) dst[HO - 1] = p.y;
+            for (int e = 0; e < E; e++) {
+                const int m = t + TPL * e;
+                typename vec2<T>::type p;
+                p.x = v[e].re * scale;
+                p.y = v[e].im * scale;
+                *reinterpret_cast<typename vec2<T>::type*>(dst + HO + 2 * m) = p;
+                if (HO > 0) {
+                    if (m == 0) dst[HO + 2 * L] = p.x;
+                    if (m == L - 1) dst[HO - 1] = p.y;
+                }
+            }
+            // the rest of the row is zeroed, so every 128-byte line of the
+            // row is written whole
+            if (HO == 0)
+                for (long z = 2 * L + t; z < rs; z += TPL) dst[z] = 0;
+            else {
+                for (int z = t; z < HO - 1; z += TPL) dst[z] = 0;
+                for (long z = HO + 2 * L + 1 + t; z < rs; z += TPL) dst[z] = 0;
             }
         }
-        if (HO == 0)
-            for (long z = 2 * L + t; z < rs; z += TPL) dst[z] = 0;
+        // the last exchange's LDS reads precede the next group's staging
+        __syncthreads();
     }
 }
 
@@ -203,6 +239,8 @@ struct RealRowLoader {
     }
 };
 
+// Each work-group walks row groups blockIdx.x, blockIdx.x + gridDim.x, ...
+// and loads the next group's row while transforming the current one.
 template <typename T, int L, int E, int RW, typename LOADER, typename OP, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
     k_zr2c(LOADER loader, cx<T>* __restrict__ out, long nrows, int N, long hs, long out_batch,
@@ -213,38 +251,56 @@ __global__ void __launch_bounds__(L / E * RW)
     __shared__ T lim[M::SIZE];
     const int t = threadIdx.x % TPL;
     const int rl = threadIdx.x / TPL;
-    const long row = (long)blockIdx.x * RW + rl;
-    const bool ok = row < nrows;
-    cx<T> v[E];
+    const long ngroups = (nrows + RW - 1) / RW;
+    cx<T> nxt[E];
+    auto load = [&](long g) {
+        const long row = g * RW + rl;
+        const bool ok = row < nrows;
 #pragma unroll
-    for (int e = 0; e < E; e++) v[e] = ok ? loader(blockIdx.y, row, t + TPL * e) : mk<T>(0, 0);
-    line_fft<T, L, E, RW, false, -1, Rs...>(v, lre, lim, tw, t, rl);
+        for (int e = 0; e < E; e++)
+            nxt[e] = ok ? loader(blockIdx.y, row, t + TPL * e) : mk<T>(0, 0);
+    };
+    if ((long)blockIdx.x < ngroups) load(blockIdx.x);
+    for (long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const long row = g * RW + rl;
+        const bool ok = row < nrows;
+        cx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int m = t + TPL * e;
-        lre[M::idx(m, rl)] = v[e].re;
-        lim[M::idx(m, rl)] = v[e].im;
-    }
-    __syncthreads();
-    if (!ok) return;
-    const int gx = (int)(row / N), gy = (int)(row % N);
-    cx<T>* dst = out + blockIdx.y * out_batch + row * hs;
-    const T h = (T)0.5;
+        for (int e = 0; e < E; e++) v[e] = nxt[e];
+        if (g + gridDim.x < ngroups) load(g + gridDim.x);
+        line_fft<T, L, E, RW, false, -1, Rs...>(v, lre, lim, tw, t, rl);
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int k = t + TPL * e;
-        const int km = (k == 0) ? 0 : L - k;
-        cx<T> zk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
-        cx<T> zm = conjg(mk<T>(lre[M::idx(km, rl)], lim[M::idx(km, rl)]));
-        cx<T> ev = cscale(zk + zm, h);
-        cx<T> od = cscale(zk - zm, h);                 // = i * O[k]
-        cx<T> w = twN[k];                              // exp(-2 pi i k / N)
-        cx<T> xk = ev + mul_i<-1>(cmul(w, od));        // E + w * O
-        st(dst + k, op.post(xk, gx, gy, k));
-    }
-    if (t == 0) {
-        cx<T> z0 = mk<T>(lre[M::idx(0, rl)], lim[M::idx(0, rl)]);
-        st(dst + L, op.post(mk<T>(z0.re - z0.im, (T)0), gx, gy, L));
+        for (int e = 0; e < E; e++) {
+            const int m = t + TPL * e;
+            lre[M::idx(m, rl)] = v[e].re;
+            lim[M::idx(m, rl)] = v[e].im;
+        }
+        __syncthreads();
+        if (ok) {
+            const int gx = (int)(row / N), gy = (int)(row % N);
+            cx<T>* dst = out + blockIdx.y * out_batch + row * hs;
+            const T h = (T)0.5;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + TPL * e;
+                const int km = (k == 0) ? 0 : L - k;
+                cx<T> zk = mk<T>(lre[M::idx(k, rl)], lim[M::idx(k, rl)]);
+                cx<T> zm = conjg(mk<T>(lre[M::idx(km, rl)], lim[M::idx(km, rl)]));
+                cx<T> ev = cscale(zk + zm, h);
+                cx<T> od = cscale(zk - zm, h);           // = i * O[k]
+                cx<T> w = twN[k];                        // exp(-2 pi i k / N)
+                cx<T> xk = ev + mul_i<-1>(cmul(w, od));  // E + w * O
+                st(dst + k, op.post(xk, gx, gy, k));
+            }
+            if (t == 0) {
+                cx<T> z0 = mk<T>(lre[M::idx(0, rl)], lim[M::idx(0, rl)]);
+                st(dst + L, op.post(mk<T>(z0.re - z0.im, (T)0), gx, gy, L));
+            }
+            // zeros in the row padding: whole 128-byte lines are written
+            for (long k = L + 1 + t; k < hs; k += TPL) st(dst + k, mk<T>(0, 0));
+        }
+        // the LDS reads above precede the next group's exchanges
+        __syncthreads();
     }
 }
 

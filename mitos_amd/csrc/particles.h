@@ -180,12 +180,11 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- displacement grids with a periodic z halo -------------------------------------
-// The pipeline stores each displacement grid as rows of rs = N + 8 values:
-// value (x, y, z) at row (x*N + y) offset MITOS_PSI_HO + z, with the halo
-// values z = -1 and z = N (copies of z = N-1 and z = 0) beside the row, so the
-// three z taps of a TSC stencil are always three consecutive values.
+// The pipeline stores each displacement grid as rows of rs >= N + MITOS_PSI_HO + 1
+// values: value (x, y, z) at row (x*N + y) offset MITOS_PSI_HO + z, with the
+// halo values z = -1 and z = N (copies of z = N-1 and z = 0) beside the row, so
+// the three z taps of a TSC stencil are always three consecutive values.
 #define MITOS_PSI_HO 4
-#define MITOS_PSI_PAD 8
 
 // FP32 TSC weights: nearest cell c = floor(X + 1/2), u = X - c in [-1/2, 1/2],
 // weights of cells c-1, c, c+1.  Same cells and (up to rounding) the same

@@ -374,7 +374,7 @@ struct Pipe {
                     }, &sc.ms[TUNE_X_FWD][v]));
                     MITOS_TRY(time_it([&] {
                         return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(
-                            v, b(0), psi(0), hs, NN + MITOS_PSI_PAD, 1, 0, 0, (T)1, OpNone(), s);
+                            v, b(0), psi(0), hs, prs(), 1, 0, 0, (T)1, OpNone(), s);
                     }, &sc.ms[TUNE_Z_C2R][v]));
                     MITOS_TRY(time_it([&] {
                         return zr2c_pass_tuned<T, NN>(v, ld, K, hs, 1, 0, OpNone(), s);
@@ -394,7 +394,7 @@ struct Pipe {
         return MITOS_OK;
     }
 
-    // Mean time of a plain 3D r2c (real rows of N + MITOS_PSI_PAD -> K) and c2r
+    // Mean time of a plain 3D r2c (real rows of prs() -> K) and c2r
     // (K -> real) over reps runs each, with the tuned schedules; uses the
     // pipeline's grids as scratch.
     int fft_timing(hipStream_t s, int reps, float* r2c_ms, float* c2r_ms) {
@@ -403,7 +403,7 @@ struct Pipe {
         MITOS_HIP_CHECK(hipEventCreate(&e1));
         MITOS_HIP_CHECK(hipEventCreate(&e2));
         T* real = psi(0);
-        const long rs = N + MITOS_PSI_PAD;
+        const long rs = prs();
         const int rc = with_size(N, [&]<int NN>() -> int {
             RealRowLoader<T> ld{real, rs, 0};
             auto r2c = [&]() -> int {
@@ -444,6 +444,9 @@ struct Pipe {
     cx<T>* a(int i) { return A + (size_t)i * csize; }
     cx<T>* b(int i) { return B + (size_t)i * csize; }
     T* psi(int i) { return reinterpret_cast<T*>(b(3 + i)); }
+    // Row stride (reals) of the displacement and delta2 grids: one complex row,
+    // so every real row starts on a 128-byte boundary as the complex rows do.
+    long prs() const { return 2 * hs; }
     unsigned long long* cic() { return reinterpret_cast<unsigned long long*>(b(0)); }
     // fixed-point unit of the lattice deposit (float weights in the FP32 pipeline)
     static int cic_shift() { return sizeof(T) == 4 ? MITOS_CIC_SHIFT_F32 : MITOS_CIC_SHIFT; }
@@ -454,8 +457,9 @@ struct Pipe {
         const long q = 128 / sizeof(cx<T>);
         hs = (H + q - 1) / q * q;
         csize = (long)N * N * hs;
-        // a displacement grid (rows of N + MITOS_PSI_PAD reals) fits in a complex grid
-        if (N + MITOS_PSI_PAD > 2 * hs) return MITOS_ERR_UNSUPPORTED;
+        // a displacement grid (rows of prs() = 2 hs reals, halo included) fits
+        // in a complex grid
+        if (N + MITOS_PSI_HO + 1 > 2 * hs) return MITOS_ERR_UNSUPPORTED;
         const long long np = (long long)N * N * N;
         MITOS_HIP_CHECK(hipMalloc(&K, sizeof(cx<T>) * csize));
         MITOS_HIP_CHECK(hipMalloc(&A, sizeof(cx<T>) * csize * 3));
@@ -593,7 +597,7 @@ struct Pipe {
                 p4.out[0] = D;
                 // delta2(x) rows into a(1) as reals, then their r2c into D
                 T* d2 = reinterpret_cast<T*>(a(1));
-                const long rs2 = NN + MITOS_PSI_PAD;
+                const long rs2 = prs();
                 MITOS_TRY(timed(s, "hessian_z_delta2", 6 * G + Rg, [&] {
                     return HessZK<T, NN>::go(p4, hs, dk, (T)(1.0 / boxvol), d2, rs2, s);
                 }));
@@ -656,17 +660,17 @@ struct Pipe {
             const T cs = (T)(1.0 / boxvol);
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
                 return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(0), psi(0), hs,
-                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            prs(), 1, 0, 0, cs,
                                                             OpNone(), s);
             }));
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
                 return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(1), psi(1), hs,
-                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            prs(), 1, 0, 0, cs,
                                                             OpNone(), s);
             }));
             MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
                 return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(2), psi(2), hs,
-                                                            NN + MITOS_PSI_PAD, 1, 0, 0, cs,
+                                                            prs(), 1, 0, 0, cs,
                                                             OpIkz{dk}, s);
             }));
             mark(4);
@@ -690,7 +694,7 @@ struct Pipe {
                 // the forward of the adjoint: positions stay in T up to the deposit
                 MITOS_TRY(timed(s, "move_deposit", 3 * Rg + 12.0 * np + 8.0 * np, [&] {
                     hipLaunchKernelGGL((k_move_deposit_s<T>), dim3(nblk(np)), dim3(256), 0, s,
-                                       psi(0), psi(1), psi(2), (unsigned)(NN + MITOS_PSI_PAD),
+                                       psi(0), psi(1), psi(2), (unsigned)(prs()),
                                        NN, L, spacing, X, Y, Z, cic(), cic_shift());
                     return launch_status();
                 }));
@@ -700,7 +704,7 @@ struct Pipe {
                     constexpr int NT = NN >= 256 ? 256 : 64;
                     const int nch = (NN + NT - 1) / NT;
                     const dim3 grid((unsigned)((long)NN * NN * nch));
-                    const unsigned rs = (unsigned)(NN + MITOS_PSI_PAD);
+                    const unsigned rs = (unsigned)(prs());
                     if (STENCIL && NN % 64 == 0) {
                         constexpr int TX = 16, PPT = 8;
                         const dim3 gridw((unsigned)((long)NN * NN / PPT * (NN / 64)));

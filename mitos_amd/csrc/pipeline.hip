@@ -253,12 +253,12 @@ struct OpUnstencil {
 
 // dense (3, N, N, N) copy of the halo-padded displacement grids
 template <typename T>
-__global__ void k_copy_psi(const T* src, long src_stride, T* dst, long n, int N) {
+__global__ void k_copy_psi(const T* src, long src_stride, long rs, T* dst, long n, int N) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * n) return;
     const long g = i / n, j = i % n;
     const long row = j / N, z = j % N;
-    dst[i] = src[g * src_stride + row * (N + MITOS_PSI_PAD) + MITOS_PSI_HO + z];
+    dst[i] = src[g * src_stride + row * rs + MITOS_PSI_HO + z];
 }
 
 // FP32: the grids hold S_xyz psi_x, S_yz psi_y, S_z psi_z (see
@@ -268,7 +268,7 @@ template <typename T>
 static int unstencil_psi(Pipe<T>* P, int N, T* out, hipStream_t s) {
     const double* S = smooth_table<double>(N);
     if (!S) return MITOS_ERR_ALLOC;
-    const GridLayout gin{N, P->hs, (long)N + MITOS_PSI_PAD};
+    const GridLayout gin{N, P->hs, P->prs()};
     const GridLayout gout{N, P->hs, (long)N};
     const double norm = 1.0 / ((double)N * N * N);
     const int masks[3] = {7, 6, 4};
@@ -291,10 +291,10 @@ extern "C" int mitos_ic_displacements(mitos_ic_context* c, void* psi) {
         MITOS_TRY(set_last_error(unstencil_psi(c->pd, c->N, (double*)psi, c->stream)));
     else if (c->pf)
         hipLaunchKernelGGL((k_copy_psi<float>), dim3(nblk(3 * n)), dim3(256), 0, c->stream,
-                           c->pf->psi(0), 2 * c->pf->csize, (float*)psi, n, c->N);
+                           c->pf->psi(0), 2 * c->pf->csize, c->pf->prs(), (float*)psi, n, c->N);
     else
         hipLaunchKernelGGL((k_copy_psi<double>), dim3(nblk(3 * n)), dim3(256), 0, c->stream,
-                           c->pd->psi(0), 2 * c->pd->csize, (double*)psi, n, c->N);
+                           c->pd->psi(0), 2 * c->pd->csize, c->pd->prs(), (double*)psi, n, c->N);
     MITOS_TRY(set_last_error(launch_status()));
     MITOS_HIP_CHECK(hipStreamSynchronize(c->stream));
     return MITOS_OK;

@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(L / E * C)
     const int col = bx * C + c;
     const int o = by;
     const bool ok = col < ncols;
+    const bool wok = col < stored_cols<C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = (long)o * ostride + col;
@@ -86,7 +87,8 @@ __global__ void __launch_bounds__(L / E * C)
             u[e] = apply_mul(CODES::mul(j), v[e], k);
         }
         line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-        if (ok) {
+        // padding columns of the last tile hold zeros (zero input, linear map)
+        if (wok) {
             cx<T>* out = P.out[j];
 #pragma unroll
             for (int e = 0; e < E; e++) st(out + base + (long)(t + TPL * e) * pstride, u[e]);
@@ -222,6 +224,7 @@ __global__ void __launch_bounds__(L / E * C)
     const int col = bx * C + c;
     const int y = by;
     const bool ok = col < ncols;
+    const bool wok = col < stored_cols<C>(ncols, hs);
     const long pstride = (long)L * hs;
     const long base = (long)y * hs + col;
     cx<T> v[E];
@@ -273,13 +276,14 @@ __global__ void __launch_bounds__(L / E * C)
         for (int e = 0; e < E; e++)
             u[e] = apply_mul(1, v[e], (T)(freq_index(t + TPL * e, L) * cp.dk));
     }
+    // padding columns of the last tile hold zeros (zero input, linear map)
     line_fft<T, L, E, C, true, 1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-    if (ok) {
+    if (wok) {
 #pragma unroll
         for (int e = 0; e < E; e++) st(C0 + base + (long)(t + TPL * e) * pstride, v[e]);
     }
     line_fft<T, L, E, C, true, 1, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-    if (ok) {
+    if (wok) {
 #pragma unroll
         for (int e = 0; e < E; e++) st(C1 + base + (long)(t + TPL * e) * pstride, u[e]);
     }
