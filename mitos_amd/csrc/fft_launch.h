@@ -85,15 +85,22 @@ inline int launch_status() {
 }
 
 // ---- strided pass ----------------------------------------------------------
+// k_pass_strided addresses a thread's elements by a 32-bit byte offset from a
+// uniform base: TPL line positions of stride pstride must span under 4 GiB
+template <typename T, int L, int AXIS>
+inline bool strided_offsets_fit(int tpl, long hs) {
+    const long pstride = AXIS == 0 ? (long)L * hs : hs;
+    return (long)tpl * pstride * (long)sizeof(cx<T>) < (1L << 32);
+}
+
 template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, typename RAD>
 struct StridedK;
 template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, int... Rs>
 struct StridedK<T, L, C, AXIS, DIR, SPLIT, OP, Radices<Rs...>> {
     static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
                    int ncols, long bstride, const cx<T>* tw, OP op) {
-        hipLaunchKernelGGL(
-            (k_pass_strided<T, L, LineCfg<L>::E, C, AXIS, DIR, SPLIT, OP, Rs...>), g, b, 0, s,
-            in, out, hs, ncols, bstride, tw, op);
+        hipLaunchKernelGGL((k_pass_strided<T, L, LineCfg<L>::E, C, AXIS, DIR, SPLIT, OP, Rs...>),
+                           g, b, 0, s, in, out, hs, ncols, bstride, tw, op, 0);
     }
 };
 
@@ -105,6 +112,7 @@ int strided_pass(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride,
     constexpr int TPL = L / E;
     constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
     const int ncols = L / 2 + 1;
+    if (!strided_offsets_fit<T, L, AXIS>(TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
     dim3 grid((ncols + C - 1) / C, L, nbatch);
@@ -142,8 +150,9 @@ int zc2r_pass(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, l
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
     dim3 grid((unsigned)((nrows + RW - 1) / RW), nbatch);
-    ZC2RK<T, L, RW, HO, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, in, out, nrows,
-                                                     N, hs, rs, inb, outb, tw, twN, scale, op);
+    ZC2RK<T, L, RW, HO, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, in, out, 0L,
+                                                     nrows, N, hs, rs, inb, outb, tw, twN, scale,
+                                                     op);
     return launch_status();
 }
 
@@ -169,7 +178,7 @@ int zr2c_pass(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op, 
     if (!tw || !twN) return MITOS_ERR_ALLOC;
     dim3 grid((unsigned)((nrows + RW - 1) / RW), nbatch);
     ZR2CK<T, L, RW, LOADER, OP, typename LineCfg<L>::R>::go(grid, dim3(TPL * RW), s, loader,
-                                                             out, nrows, N, hs, outb, tw, twN,
+                                                             out, 0L, nrows, N, hs, outb, tw, twN,
                                                              op);
     return launch_status();
 }
@@ -225,37 +234,48 @@ template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, 
           int... Rs>
 struct StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, CFG, Radices<Rs...>> {
     static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
-                   int ncols, long bstride, const cx<T>* tw, OP op) {
+                   int ncols, long bstride, const cx<T>* tw, OP op, int o0) {
         hipLaunchKernelGGL((k_pass_strided<T, L, CFG::E, C, AXIS, DIR, SPLIT, OP, Rs...>), g, b, 0,
-                           s, in, out, hs, ncols, bstride, tw, op);
+                           s, in, out, hs, ncols, bstride, tw, op, o0);
     }
 };
 
+// outer indices o0 .. o0+no-1 only when no > 0
 template <typename T, int L, int AXIS, int DIR, int V, typename OP>
 int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride, OP op,
-                   hipStream_t s) {
+                   hipStream_t s, int o0 = 0, int no = 0) {
+    if (no <= 0) {
+        o0 = 0;
+        no = L;
+    }
+    if (o0 < 0 || o0 + no > L) return MITOS_ERR_ARG;
     typedef StridedSched<T, L, V> S;
     constexpr int C = S::C;
     constexpr long LDS2 = 2 * (long)LdsMap<T, L, C, true>::SIZE * (long)sizeof(T);
     constexpr bool SPLIT = V == 3 ? LDS2 > 163840 : LDS2 > 81920;
     const int ncols = L / 2 + 1;
+    if (!strided_offsets_fit<T, L, AXIS>(S::TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
-    dim3 grid((ncols + C - 1) / C, L, nbatch);
-    StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, typename S::CFG>::go(grid, dim3(S::TPL * C), s, in,
-                                                                  out, hs, ncols, bstride, tw, op);
+    dim3 grid((ncols + C - 1) / C, no, nbatch);
+    StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, typename S::CFG>::go(
+        grid, dim3(S::TPL * C), s, in, out, hs, ncols, bstride, tw, op, o0);
     return launch_status();
 }
 
 // strided pass with a run-time schedule variant
 template <typename T, int L, int AXIS, int DIR, typename OP>
 int strided_pass_tuned(int variant, const cx<T>* in, cx<T>* out, long hs, int nbatch,
-                       long bstride, OP op, hipStream_t s) {
+                       long bstride, OP op, hipStream_t s, int o0 = 0, int no = 0) {
     switch (variant) {
-        case 1: return strided_pass_v<T, L, AXIS, DIR, 1>(in, out, hs, nbatch, bstride, op, s);
-        case 2: return strided_pass_v<T, L, AXIS, DIR, 2>(in, out, hs, nbatch, bstride, op, s);
-        case 3: return strided_pass_v<T, L, AXIS, DIR, 3>(in, out, hs, nbatch, bstride, op, s);
-        default: return strided_pass_v<T, L, AXIS, DIR, 0>(in, out, hs, nbatch, bstride, op, s);
+        case 1:
+            return strided_pass_v<T, L, AXIS, DIR, 1>(in, out, hs, nbatch, bstride, op, s, o0, no);
+        case 2:
+            return strided_pass_v<T, L, AXIS, DIR, 2>(in, out, hs, nbatch, bstride, op, s, o0, no);
+        case 3:
+            return strided_pass_v<T, L, AXIS, DIR, 3>(in, out, hs, nbatch, bstride, op, s, o0, no);
+        default:
+            return strided_pass_v<T, L, AXIS, DIR, 0>(in, out, hs, nbatch, bstride, op, s, o0, no);
     }
 }
 
@@ -288,31 +308,44 @@ struct ZC2RKV<T, L, RW, HO, OP, CFG, Radices<Rs...>> {
     }
 };
 
+// rows r0 .. r1-1 only when r1 > 0
 template <typename T, int N, int V, int HO, typename OP>
 int zc2r_pass_v(const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb, long outb,
-                T scale, OP op, hipStream_t s) {
+                T scale, OP op, hipStream_t s, long r0 = 0, long r1 = 0) {
     if (rs < N + (HO > 0 ? HO + 1 : 0)) return MITOS_ERR_ARG;
     constexpr int L = N / 2;
     typedef ZSched<T, L, V> S;
-    const long nrows = (long)N * N;
+    if (r1 <= 0) {
+        r0 = 0;
+        r1 = (long)N * N;
+    }
+    if (r0 < 0 || r1 > (long)N * N || r0 >= r1) return MITOS_ERR_ARG;
     const cx<T>* tw = twiddles<T>(L);
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
-    dim3 grid(z_groups(nrows, S::RW, S::TPL), nbatch);
-    ZC2RKV<T, L, S::RW, HO, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, in, out,
-                                                     nrows, N, hs, rs, inb, outb, tw, twN, scale,
+    dim3 grid(z_groups(r1 - r0, S::RW, S::TPL), nbatch);
+    ZC2RKV<T, L, S::RW, HO, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, in, out, r0,
+                                                     r1, N, hs, rs, inb, outb, tw, twN, scale,
                                                      op);
     return launch_status();
 }
 
 template <typename T, int N, int HO, typename OP>
 int zc2r_pass_tuned(int variant, const cx<T>* in, T* out, long hs, long rs, int nbatch, long inb,
-                    long outb, T scale, OP op, hipStream_t s) {
+                    long outb, T scale, OP op, hipStream_t s, long r0 = 0, long r1 = 0) {
     switch (variant) {
-        case 1: return zc2r_pass_v<T, N, 1, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
-        case 2: return zc2r_pass_v<T, N, 2, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
-        case 3: return zc2r_pass_v<T, N, 3, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
-        default: return zc2r_pass_v<T, N, 0, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s);
+        case 1:
+            return zc2r_pass_v<T, N, 1, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s, r0,
+                                            r1);
+        case 2:
+            return zc2r_pass_v<T, N, 2, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s, r0,
+                                            r1);
+        case 3:
+            return zc2r_pass_v<T, N, 3, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s, r0,
+                                            r1);
+        default:
+            return zc2r_pass_v<T, N, 0, HO>(in, out, hs, rs, nbatch, inb, outb, scale, op, s, r0,
+                                            r1);
     }
 }
 
@@ -327,29 +360,34 @@ struct ZR2CKV<T, L, RW, LOADER, OP, CFG, Radices<Rs...>> {
     }
 };
 
+// rows r0 .. r1-1 only when r1 > 0
 template <typename T, int N, int V, typename LOADER, typename OP>
 int zr2c_pass_v(LOADER loader, cx<T>* out, long hs, int nbatch, long outb, OP op,
-                hipStream_t s) {
+                hipStream_t s, long r0 = 0, long r1 = 0) {
     constexpr int L = N / 2;
     typedef ZSched<T, L, V> S;
-    const long nrows = (long)N * N;
+    if (r1 <= 0) {
+        r0 = 0;
+        r1 = (long)N * N;
+    }
+    if (r0 < 0 || r1 > (long)N * N || r0 >= r1) return MITOS_ERR_ARG;
     const cx<T>* tw = twiddles<T>(L);
     const cx<T>* twN = twiddles<T>(N);
     if (!tw || !twN) return MITOS_ERR_ALLOC;
-    dim3 grid(z_groups(nrows, S::RW, S::TPL), nbatch);
+    dim3 grid(z_groups(r1 - r0, S::RW, S::TPL), nbatch);
     ZR2CKV<T, L, S::RW, LOADER, OP, typename S::CFG>::go(grid, dim3(S::TPL * S::RW), s, loader,
-                                                         out, nrows, N, hs, outb, tw, twN, op);
+                                                         out, r0, r1, N, hs, outb, tw, twN, op);
     return launch_status();
 }
 
 template <typename T, int N, typename LOADER, typename OP>
 int zr2c_pass_tuned(int variant, LOADER loader, cx<T>* out, long hs, int nbatch, long outb,
-                    OP op, hipStream_t s) {
+                    OP op, hipStream_t s, long r0 = 0, long r1 = 0) {
     switch (variant) {
-        case 1: return zr2c_pass_v<T, N, 1>(loader, out, hs, nbatch, outb, op, s);
-        case 2: return zr2c_pass_v<T, N, 2>(loader, out, hs, nbatch, outb, op, s);
-        case 3: return zr2c_pass_v<T, N, 3>(loader, out, hs, nbatch, outb, op, s);
-        default: return zr2c_pass_v<T, N, 0>(loader, out, hs, nbatch, outb, op, s);
+        case 1: return zr2c_pass_v<T, N, 1>(loader, out, hs, nbatch, outb, op, s, r0, r1);
+        case 2: return zr2c_pass_v<T, N, 2>(loader, out, hs, nbatch, outb, op, s, r0, r1);
+        case 3: return zr2c_pass_v<T, N, 3>(loader, out, hs, nbatch, outb, op, s, r0, r1);
+        default: return zr2c_pass_v<T, N, 0>(loader, out, hs, nbatch, outb, op, s, r0, r1);
     }
 }
 

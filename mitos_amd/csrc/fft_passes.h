@@ -87,14 +87,15 @@ __device__ __forceinline__ int stored_cols(int ncols, long hs) {
 }
 
 // ---- strided c2c pass ------------------------------------------------------
-// AXIS 0: lines along x, blockIdx.y = y;  AXIS 1: lines along y, blockIdx.y = x.
-// blockIdx.z selects one of several grids spaced `batch_stride` apart.
-// in == out is allowed: every thread stores exactly the elements it loaded.
+// AXIS 0: lines along x, outer index o = y;  AXIS 1: lines along y, o = x.
+// blockIdx.y selects o = o0 + blockIdx.y; blockIdx.z one of several grids
+// spaced `batch_stride` apart.  in == out is allowed: every thread stores
+// exactly the elements it loaded.
 template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, typename OP,
           int... Rs>
 __global__ void __launch_bounds__(L / E * C)
-    k_pass_strided(const cx<T>* in, cx<T>* out, long hs,
-                   int ncols, long batch_stride, const cx<T>* __restrict__ tw, OP op) {
+    k_pass_strided(const cx<T>* in, cx<T>* out, long hs, int ncols, long batch_stride,
+                   const cx<T>* __restrict__ tw, OP op, int o0) {
     constexpr int TPL = L / E;
     typedef LdsMap<T, L, C, true> M;
     __shared__ T lds[(SPLIT ? 1 : 2) * M::SIZE];
@@ -103,17 +104,24 @@ __global__ void __launch_bounds__(L / E * C)
     unsigned bx, by;
     xcd_block2(bx, by);
     const int col = bx * C + c;
-    const int o = by;
+    const int o = o0 + (int)by;
     const bool ok = col < ncols;
     const bool wok = col < stored_cols<C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = blockIdx.z * batch_stride + (long)o * ostride + col;
+    // loads address a work-group-uniform base per element plus one 32-bit
+    // per-thread offset (the launcher checks TPL * pstride elements fit in 32
+    // bits): no 64-bit address registers per element
+    const char* ub = reinterpret_cast<const char*>(in + blockIdx.z * batch_stride +
+                                                   (long)o * ostride + (long)bx * C);
+    const unsigned toff = (unsigned)(((long)t * pstride + c) * (long)sizeof(cx<T>));
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int p = t + TPL * e;
-        cx<T> a = ok ? ld(in + base + (long)p * pstride) : mk<T>(0, 0);
+        const char* eb = ub + (long)TPL * e * pstride * (long)sizeof(cx<T>);
+        cx<T> a = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
         v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
     }
     line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
@@ -132,35 +140,39 @@ __global__ void __launch_bounds__(L / E * C)
 // in: rows of hs complex (first N/2+1 used); out: rows of rs reals.  HO = 0:
 // the N values at row[0..N), the remaining rs-N set to zero.  HO > 0: value z
 // at row[HO + z] with the periodic halo row[HO-1] = value N-1 and row[HO+N] =
-// value 0 (rs >= N + HO + 1; the other padding is left unwritten).  L = N/2.
-// Each work-group walks row groups blockIdx.x, blockIdx.x + gridDim.x, ...
-// and loads the next group's row while transforming the current one.
+// value 0 (rs >= N + HO + 1; the rest of the row is set to zero).  L = N/2.
+// Rows r0 .. r1-1 are transformed.  Each work-group walks row groups
+// blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next group's row while
+// transforming the current one.
 template <typename T, int L, int E, int RW, int HO, typename OP, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
-    k_zc2r(const cx<T>* __restrict__ in, T* __restrict__ out, long nrows, int N, long hs,
-           long rs, long in_batch, long out_batch, const cx<T>* __restrict__ tw,
-           const cx<T>* __restrict__ twN, T scale, OP op) {
+    k_zc2r(const cx<T>* __restrict__ in, T* __restrict__ out, long r0, long r1, int N, long hs,
+           long rs, long in_batch, long out_batch, const cx<T>* __restrict__ tw0,
+           const cx<T>* __restrict__ twN0, T scale, OP op) {
     constexpr int TPL = L / E;
     typedef LdsMap<T, L, RW, false> M;
     __shared__ T lre[M::SIZE];
     __shared__ T lim[M::SIZE];
-    const int t = threadIdx.x % TPL;
-    const int rl = threadIdx.x / TPL;
-    const long ngroups = (nrows + RW - 1) / RW;
+    const int t0 = threadIdx.x % TPL;
+    const int rl0 = threadIdx.x / TPL;
+    const long ngroups = (r1 - r0 + RW - 1) / RW;
     cx<T> nxt[E + 1];
     auto load = [&](long g) {
-        const long row = g * RW + rl;
-        if (row < nrows) {
+        const long row = r0 + g * RW + rl0;
+        if (row < r1) {
             const cx<T>* src = in + blockIdx.y * in_batch + row * hs;
 #pragma unroll
-            for (int e = 0; e < E; e++) nxt[e] = ld(src + t + TPL * e);
-            nxt[E] = t == 0 ? ld(src + L) : mk<T>(0, 0);
+            for (int e = 0; e < E; e++) nxt[e] = ld(src + t0 + TPL * e);
+            nxt[E] = t0 == 0 ? ld(src + L) : mk<T>(0, 0);
         }
     };
     if ((long)blockIdx.x < ngroups) load(blockIdx.x);
     for (long g = blockIdx.x; g < ngroups; g += gridDim.x) {
-        const long row = g * RW + rl;
-        const bool ok = row < nrows;
+        const int t = t0, rl = rl0;
+        const cx<T>* tw = tw0;
+        const cx<T>* twN = twN0;
+        const long row = r0 + g * RW + rl;
+        const bool ok = row < r1;
         const int gx = (int)(row / N), gy = (int)(row % N);
         // stage the row (positions 0..L) in LDS
         if (ok) {
@@ -239,31 +251,35 @@ struct RealRowLoader {
     }
 };
 
-// Each work-group walks row groups blockIdx.x, blockIdx.x + gridDim.x, ...
-// and loads the next group's row while transforming the current one.
+// Rows r0 .. r1-1 are transformed.  Each work-group walks row groups
+// blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next group's row while
+// transforming the current one.
 template <typename T, int L, int E, int RW, typename LOADER, typename OP, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
-    k_zr2c(LOADER loader, cx<T>* __restrict__ out, long nrows, int N, long hs, long out_batch,
-           const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN, OP op) {
+    k_zr2c(LOADER loader, cx<T>* __restrict__ out, long r0, long r1, int N, long hs, long out_batch,
+           const cx<T>* __restrict__ tw0, const cx<T>* __restrict__ twN0, OP op) {
     constexpr int TPL = L / E;
     typedef LdsMap<T, L, RW, false> M;
     __shared__ T lre[M::SIZE];
     __shared__ T lim[M::SIZE];
-    const int t = threadIdx.x % TPL;
-    const int rl = threadIdx.x / TPL;
-    const long ngroups = (nrows + RW - 1) / RW;
+    const int t0 = threadIdx.x % TPL;
+    const int rl0 = threadIdx.x / TPL;
+    const long ngroups = (r1 - r0 + RW - 1) / RW;
     cx<T> nxt[E];
     auto load = [&](long g) {
-        const long row = g * RW + rl;
-        const bool ok = row < nrows;
+        const long row = r0 + g * RW + rl0;
+        const bool ok = row < r1;
 #pragma unroll
         for (int e = 0; e < E; e++)
-            nxt[e] = ok ? loader(blockIdx.y, row, t + TPL * e) : mk<T>(0, 0);
+            nxt[e] = ok ? loader(blockIdx.y, row, t0 + TPL * e) : mk<T>(0, 0);
     };
     if ((long)blockIdx.x < ngroups) load(blockIdx.x);
     for (long g = blockIdx.x; g < ngroups; g += gridDim.x) {
-        const long row = g * RW + rl;
-        const bool ok = row < nrows;
+        const int t = t0, rl = rl0;
+        const cx<T>* tw = tw0;
+        const cx<T>* twN = twN0;
+        const long row = r0 + g * RW + rl;
+        const bool ok = row < r1;
         cx<T> v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) v[e] = nxt[e];

@@ -416,8 +416,10 @@ struct Pipe {
                                                         OpNone(), s);
             };
             auto c2r = [&]() -> int {
-                MITOS_TRY((strided_pass<T, NN, 0, 1>(K, a(0), hs, 1, 0, OpNone(), s)));
-                MITOS_TRY((strided_pass<T, NN, 1, 1>(a(0), a(1), hs, 1, 0, OpNone(), s)));
+                MITOS_TRY((strided_pass_tuned<T, NN, 0, 1>(sched.v[TUNE_X_FWD], K, a(0), hs, 1, 0,
+                                                           OpNone(), s)));
+                MITOS_TRY((strided_pass_tuned<T, NN, 1, 1>(sched.v[TUNE_Y_FWD], a(0), a(1), hs, 1,
+                                                           0, OpNone(), s)));
                 return zc2r_pass_tuned<T, NN, 0>(sched.v[TUNE_Z_C2R], a(1), real, hs, rs, 1, 0, 0,
                                                  (T)1, OpNone(), s);
             };

@@ -70,28 +70,38 @@ __global__ void __launch_bounds__(L / E * C)
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = (long)o * ostride + col;
+    // loads: uniform base per element plus one 32-bit per-thread offset
+    const long ubase = (long)o * ostride + (long)bx * C;
+    const unsigned toff = (unsigned)(((long)t * pstride + c) * (long)sizeof(cx<T>));
     cx<T> v[E], u[E];
 #pragma unroll
     for (int j = 0; j < NOUT; j++) {
+        // per-output copies: LDS addresses and twiddles shared by all outputs
+        // would otherwise stay live across the whole kernel
+        int tl = t, cl = c;
+        const cx<T>* twl = tw;
+        asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
         if (j == 0 || CODES::src(j) != CODES::src(j > 0 ? j - 1 : 0)) {
-            const cx<T>* in = P.in[CODES::src(j)];
+            const char* ib = reinterpret_cast<const char*>(P.in[CODES::src(j)] + ubase);
 #pragma unroll
-            for (int e = 0; e < E; e++)
-                v[e] = ok ? ld(in + base + (long)(t + TPL * e) * pstride) : mk<T>(0, 0);
+            for (int e = 0; e < E; e++) {
+                const char* eb = ib + (long)TPL * e * pstride * (long)sizeof(cx<T>);
+                v[e] = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
+            }
         }
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int pos = t + TPL * e;
+            const int pos = tl + TPL * e;
             T k = (T)(freq_index(pos, L) * dk);
             if (CODES::mul(j) == 3) k *= P.smooth[pos];
             u[e] = apply_mul(CODES::mul(j), v[e], k);
         }
-        line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+        line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
         // padding columns of the last tile hold zeros (zero input, linear map)
         if (wok) {
             cx<T>* out = P.out[j];
 #pragma unroll
-            for (int e = 0; e < E; e++) st(out + base + (long)(t + TPL * e) * pstride, u[e]);
+            for (int e = 0; e < E; e++) st(out + base + (long)(tl + TPL * e) * pstride, u[e]);
         }
     }
 }
@@ -107,15 +117,15 @@ __global__ void __launch_bounds__(L / E * C)
 template <typename T, int L, int E, int RW, int... Rs>
 __global__ void __launch_bounds__(L / E * RW)
     k_hess_z(MultiPtrs<T, 6, 1> P, long nrows, long hs, double dk, T scale,
-             const cx<T>* __restrict__ tw, const cx<T>* __restrict__ twN, T* __restrict__ real,
+             const cx<T>* __restrict__ tw0, const cx<T>* __restrict__ twN0, T* __restrict__ real,
              long rs) {
     constexpr int TPL = L / E;
     typedef LdsMap<T, L, RW, false> M;
     __shared__ T lre[M::SIZE];
     __shared__ T lim[M::SIZE];
-    const int t = threadIdx.x % TPL;
-    const int rl = threadIdx.x / TPL;
-    const long row = (long)blockIdx.x * RW + rl;
+    const int t0 = threadIdx.x % TPL;
+    const int rl0 = threadIdx.x / TPL;
+    const long row = (long)blockIdx.x * RW + rl0;
     const bool ok = row < nrows;
     // xx -> A; yy -> B = xx + yy, A = xx * yy; zz -> A = zz * B + A; then
     // A -= h^2 for xy, yz, xz: two accumulator arrays besides the transform
@@ -129,12 +139,15 @@ __global__ void __launch_bounds__(L / E * RW)
     auto load_row = [&](int q) {
         const cx<T>* src = P.in[src_of(q)] + row * hs;
 #pragma unroll
-        for (int e = 0; e < E; e++) nxt[e] = ld(src + t + TPL * e);
-        nxt[E] = t == 0 ? ld(src + L) : mk<T>(0, 0);
+        for (int e = 0; e < E; e++) nxt[e] = ld(src + t0 + TPL * e);
+        nxt[E] = t0 == 0 ? ld(src + L) : mk<T>(0, 0);
     };
     if (ok) load_row(0);
 #pragma unroll 1
     for (int q = 0; q < 6; q++) {
+        const int t = t0, rl = rl0;
+        const cx<T>* tw = tw0;
+        const cx<T>* twN = twN0;
         const int mul_q = (q == 2) ? 2 : (q >= 4) ? 1 : 0;
         // stage the row, with its k_z multiplier, in LDS
         if (ok) {
@@ -190,7 +203,7 @@ __global__ void __launch_bounds__(L / E * RW)
     T* dst = real + row * rs;
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        const int m = t + TPL * e;
+        const int m = t0 + TPL * e;
         typename vec2<T>::type p;
         p.x = A[e].re;
         p.y = A[e].im;
@@ -276,13 +289,25 @@ __global__ void __launch_bounds__(L / E * C)
         for (int e = 0; e < E; e++)
             u[e] = apply_mul(1, v[e], (T)(freq_index(t + TPL * e, L) * cp.dk));
     }
-    // padding columns of the last tile hold zeros (zero input, linear map)
-    line_fft<T, L, E, C, true, 1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    // padding columns of the last tile hold zeros (zero input, linear map).
+    // Each transform recomputes its LDS addresses and twiddles: shared by the
+    // three they would stay live across the kernel.
+    {
+        int tl = t, cl = c;
+        const cx<T>* twl = tw;
+        asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
+        line_fft<T, L, E, C, true, 1, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
+    }
     if (wok) {
 #pragma unroll
         for (int e = 0; e < E; e++) st(C0 + base + (long)(t + TPL * e) * pstride, v[e]);
     }
-    line_fft<T, L, E, C, true, 1, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
+    {
+        int tl = t, cl = c;
+        const cx<T>* twl = tw;
+        asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
+        line_fft<T, L, E, C, true, 1, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
+    }
     if (wok) {
 #pragma unroll
         for (int e = 0; e < E; e++) st(C1 + base + (long)(t + TPL * e) * pstride, u[e]);
