@@ -93,13 +93,21 @@ inline bool strided_offsets_fit(int tpl, long hs) {
     return (long)tpl * pstride * (long)sizeof(cx<T>) < (1L << 32);
 }
 
+// outer indices per work-group of a strided pass: 2 consecutive y rows for the
+// x lines (measured: the rows of an x pass stream faster from one work-group)
+template <int L, int AXIS>
+struct OuterPerBlock {
+    static constexpr int value = (AXIS == 0 && L >= 64) ? 2 : 1;
+};
+
 template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, typename RAD>
 struct StridedK;
 template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, int... Rs>
 struct StridedK<T, L, C, AXIS, DIR, SPLIT, OP, Radices<Rs...>> {
     static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
                    int ncols, long bstride, const cx<T>* tw, OP op) {
-        hipLaunchKernelGGL((k_pass_strided<T, L, LineCfg<L>::E, C, AXIS, DIR, SPLIT, OP, Rs...>),
+        hipLaunchKernelGGL((k_pass_strided<T, L, LineCfg<L>::E, C, AXIS, DIR, SPLIT,
+                                           OuterPerBlock<L, AXIS>::value, OP, Rs...>),
                            g, b, 0, s, in, out, hs, ncols, bstride, tw, op, 0);
     }
 };
@@ -115,7 +123,7 @@ int strided_pass(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride,
     if (!strided_offsets_fit<T, L, AXIS>(TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
-    dim3 grid((ncols + C - 1) / C, L, nbatch);
+    dim3 grid((ncols + C - 1) / C, L / OuterPerBlock<L, AXIS>::value, nbatch);
     StridedK<T, L, C, AXIS, DIR, SPLIT, OP, typename LineCfg<L>::R>::go(
         grid, dim3(TPL * C), s, in, out, hs, ncols, bstride, tw, op);
     return launch_status();
@@ -235,8 +243,9 @@ template <typename T, int L, int C, int AXIS, int DIR, bool SPLIT, typename OP, 
 struct StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, CFG, Radices<Rs...>> {
     static void go(dim3 g, dim3 b, hipStream_t s, const cx<T>* in, cx<T>* out, long hs,
                    int ncols, long bstride, const cx<T>* tw, OP op, int o0) {
-        hipLaunchKernelGGL((k_pass_strided<T, L, CFG::E, C, AXIS, DIR, SPLIT, OP, Rs...>), g, b, 0,
-                           s, in, out, hs, ncols, bstride, tw, op, o0);
+        hipLaunchKernelGGL((k_pass_strided<T, L, CFG::E, C, AXIS, DIR, SPLIT,
+                                           OuterPerBlock<L, AXIS>::value, OP, Rs...>),
+                           g, b, 0, s, in, out, hs, ncols, bstride, tw, op, o0);
     }
 };
 
@@ -248,7 +257,8 @@ int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstrid
         o0 = 0;
         no = L;
     }
-    if (o0 < 0 || o0 + no > L) return MITOS_ERR_ARG;
+    constexpr int KO = OuterPerBlock<L, AXIS>::value;
+    if (o0 < 0 || o0 + no > L || o0 % KO || no % KO) return MITOS_ERR_ARG;
     typedef StridedSched<T, L, V> S;
     constexpr int C = S::C;
     constexpr long LDS2 = 2 * (long)LdsMap<T, L, C, true>::SIZE * (long)sizeof(T);
@@ -257,7 +267,7 @@ int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstrid
     if (!strided_offsets_fit<T, L, AXIS>(S::TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
-    dim3 grid((ncols + C - 1) / C, no, nbatch);
+    dim3 grid((ncols + C - 1) / C, no / KO, nbatch);
     StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, typename S::CFG>::go(
         grid, dim3(S::TPL * C), s, in, out, hs, ncols, bstride, tw, op, o0);
     return launch_status();

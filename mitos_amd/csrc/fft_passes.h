@@ -88,10 +88,12 @@ __device__ __forceinline__ int stored_cols(int ncols, long hs) {
 
 // ---- strided c2c pass ------------------------------------------------------
 // AXIS 0: lines along x, outer index o = y;  AXIS 1: lines along y, o = x.
-// blockIdx.y selects o = o0 + blockIdx.y; blockIdx.z one of several grids
+// Work-group (bx, by) transforms column tile bx for the KO outer indices
+// o = o0 + by*KO + i, i < KO (consecutive y rows of an x pass stream faster
+// from one work-group than from KO); blockIdx.z selects one of several grids
 // spaced `batch_stride` apart.  in == out is allowed: every thread stores
 // exactly the elements it loaded.
-template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, typename OP,
+template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, int KO, typename OP,
           int... Rs>
 __global__ void __launch_bounds__(L / E * C)
     k_pass_strided(const cx<T>* in, cx<T>* out, long hs, int ncols, long batch_stride,
@@ -104,34 +106,42 @@ __global__ void __launch_bounds__(L / E * C)
     unsigned bx, by;
     xcd_block2(bx, by);
     const int col = bx * C + c;
-    const int o = o0 + (int)by;
     const bool ok = col < ncols;
     const bool wok = col < stored_cols<C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
-    const long base = blockIdx.z * batch_stride + (long)o * ostride + col;
     // loads address a work-group-uniform base per element plus one 32-bit
     // per-thread offset (the launcher checks TPL * pstride elements fit in 32
     // bits): no 64-bit address registers per element
-    const char* ub = reinterpret_cast<const char*>(in + blockIdx.z * batch_stride +
-                                                   (long)o * ostride + (long)bx * C);
     const unsigned toff = (unsigned)(((long)t * pstride + c) * (long)sizeof(cx<T>));
-    cx<T> v[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        const int p = t + TPL * e;
-        const char* eb = ub + (long)TPL * e * pstride * (long)sizeof(cx<T>);
-        cx<T> a = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
-        v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
-    }
-    line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, tw, t, c);
-    if (wok) {
+#pragma unroll 1
+    for (int i = 0; i < KO; i++) {
+        const int o = o0 + (int)by * KO + i;
+        const long base = blockIdx.z * batch_stride + (long)o * ostride + col;
+        const char* ub = reinterpret_cast<const char*>(in + blockIdx.z * batch_stride +
+                                                       (long)o * ostride + (long)bx * C);
+        // per-row copies: LDS addresses and twiddles kept live across the rows
+        // would take registers from the values
+        int tl = t, cl = c;
+        const cx<T>* twl = tw;
+        if (KO > 1) asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
+        cx<T> v[E];
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int p = t + TPL * e;
-            cx<T> a = !ok ? mk<T>(0, 0)
-                          : (AXIS == 0) ? op.post(v[e], p, o, col) : op.post(v[e], o, p, col);
-            st(out + base + (long)p * pstride, a);
+            const int p = tl + TPL * e;
+            const char* eb = ub + (long)TPL * e * pstride * (long)sizeof(cx<T>);
+            cx<T> a = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
+            v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
+        }
+        line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
+        if (wok) {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int p = tl + TPL * e;
+                cx<T> a = !ok ? mk<T>(0, 0)
+                              : (AXIS == 0) ? op.post(v[e], p, o, col) : op.post(v[e], o, p, col);
+                st(out + base + (long)p * pstride, a);
+            }
         }
     }
 }
