@@ -222,8 +222,10 @@ struct CombineParams {
     const double* smooth;
 };
 
+// two 1024-thread work-groups per CU need at most 64 registers per thread
+// (float lines up to 1024 points; the others keep their registers)
 template <typename T, int L, int E, int C, bool SPLIT, int... Rs>
-__global__ void __launch_bounds__(L / E * C)
+__global__ void __launch_bounds__(L / E * C) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && L <= 1024 ? 8 : 1)))
     k_x_combine(const cx<T>* D, const cx<T>* __restrict__ K, cx<T>* C0, cx<T>* C1, long hs,
                 int ncols, CombineParams cp, const cx<T>* __restrict__ tw) {
 #pragma clang fp contract(off)
