@@ -1215,7 +1215,9 @@ __global__ void __launch_bounds__(NT)
     __shared__ int cand[MAXC][4];  // source tile and window origin relative to this tile
     __shared__ int ncand;
     const int ntz = N / PZ, nty = N / PY, ntx = N / PX;
-    const int tile = blockIdx.x;
+    // z-neighbouring output tiles read the two ends of the same window rows:
+    // consecutive tiles run on one XCD, so the shared lines come from its L2
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
     const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
     const int yt = threadIdx.x / PZ, zt = threadIdx.x % PZ;
     unsigned long long acc[PX];
