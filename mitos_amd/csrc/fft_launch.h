@@ -123,7 +123,8 @@ int strided_pass(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstride,
     if (!strided_offsets_fit<T, L, AXIS>(TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
-    dim3 grid((ncols + C - 1) / C, L / OuterPerBlock<L, AXIS>::value, nbatch);
+    dim3 grid((stored_cols<T, C>(ncols, hs) + C - 1) / C, L / OuterPerBlock<L, AXIS>::value,
+              nbatch);
     StridedK<T, L, C, AXIS, DIR, SPLIT, OP, typename LineCfg<L>::R>::go(
         grid, dim3(TPL * C), s, in, out, hs, ncols, bstride, tw, op);
     return launch_status();
@@ -267,7 +268,7 @@ int strided_pass_v(const cx<T>* in, cx<T>* out, long hs, int nbatch, long bstrid
     if (!strided_offsets_fit<T, L, AXIS>(S::TPL, hs)) return MITOS_ERR_UNSUPPORTED;
     const cx<T>* tw = twiddles<T>(L);
     if (!tw) return MITOS_ERR_ALLOC;
-    dim3 grid((ncols + C - 1) / C, no / KO, nbatch);
+    dim3 grid((stored_cols<T, C>(ncols, hs) + C - 1) / C, no / KO, nbatch);
     StridedKV<T, L, C, AXIS, DIR, SPLIT, OP, typename S::CFG>::go(
         grid, dim3(S::TPL * C), s, in, out, hs, ncols, bstride, tw, op, o0);
     return launch_status();

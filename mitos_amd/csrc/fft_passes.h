@@ -76,13 +76,15 @@ struct OpScale {
     __device__ __forceinline__ cx<T> post(cx<T> v, int, int, int) const { return cscale(v, s); }
 };
 
-// Columns a strided pass stores: the valid ones rounded up to whole tiles of C
-// while the row padding allows.  The last tile then writes whole 128-byte lines
-// (zeros in the padding columns) instead of one partial line per row, which the
-// memory system completes with an extra read.
-template <int C>
-__device__ __forceinline__ int stored_cols(int ncols, long hs) {
-    const long r = (long)(ncols + C - 1) / C * C;
+// Columns a strided pass stores: the valid ones rounded up to whole 128-byte
+// lines (or tiles of C, if wider) while the row padding allows.  The last
+// tiles then write whole lines (zeros in the padding columns) instead of one
+// partial line per row, which the memory system completes with an extra read.
+// A launch of stored_cols / C tiles writes every line whole.
+template <typename T, int C>
+__host__ __device__ __forceinline__ int stored_cols(int ncols, long hs) {
+    constexpr int W = C * (int)sizeof(cx<T>) >= 128 ? C : 128 / (int)sizeof(cx<T>);
+    const long r = (long)(ncols + W - 1) / W * W;
     return (int)(r < hs ? r : hs);
 }
 
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(L / E * C)
     xcd_block2(bx, by);
     const int col = bx * C + c;
     const bool ok = col < ncols;
-    const bool wok = col < stored_cols<C>(ncols, hs);
+    const bool wok = col < stored_cols<T, C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     // loads address a work-group-uniform base per element plus one 32-bit
@@ -130,8 +132,11 @@ __global__ void __launch_bounds__(L / E * C)
         for (int e = 0; e < E; e++) {
             const int p = tl + TPL * e;
             const char* eb = ub + (long)TPL * e * pstride * (long)sizeof(cx<T>);
+            // columns past ncols (tile and padding) are zeros; the operator
+            // (which may index tables by column) sees valid columns only
             cx<T> a = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
-            v[e] = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
+            if (ok) a = (AXIS == 0) ? op.pre(a, p, o, col) : op.pre(a, o, p, col);
+            v[e] = a;
         }
         line_fft<T, L, E, C, true, DIR, Rs...>(v, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
         if (wok) {

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(L / E * C)
     const int col = bx * C + c;
     const int o = by;
     const bool ok = col < ncols;
-    const bool wok = col < stored_cols<C>(ncols, hs);
+    const bool wok = col < stored_cols<T, C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
     const long base = (long)o * ostride + col;
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(L / E * C) __attribute__((amdgpu_waves_per_eu(
     const int col = bx * C + c;
     const int y = by;
     const bool ok = col < ncols;
-    const bool wok = col < stored_cols<C>(ncols, hs);
+    const bool wok = col < stored_cols<T, C>(ncols, hs);
     const long pstride = (long)L * hs;
     const long base = (long)y * hs + col;
     cx<T> v[E];
