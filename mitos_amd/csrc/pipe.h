@@ -211,9 +211,10 @@ struct MultiK<T, L, AXIS, DIR, NIN, CODES, Radices<Rs...>> {
         const int ncols = L / 2 + 1;
         const cx<T>* tw = twiddles<T>(L);
         if (!tw) return MITOS_ERR_ALLOC;
-        hipLaunchKernelGGL((k_multi_strided<T, L, E, C, AXIS, DIR, SPLIT, NIN, CODES, Rs...>),
-                           dim3((ncols + C - 1) / C, L), dim3(L / E * C), 0, s, P, hs, ncols, dk,
-                           tw);
+        constexpr int KO = (AXIS == 0 && L >= 64) ? 2 : 1;
+        hipLaunchKernelGGL((k_multi_strided<T, L, E, C, AXIS, DIR, SPLIT, NIN, CODES, KO, Rs...>),
+                           dim3((ncols + C - 1) / C, L / KO), dim3(L / E * C), 0, s, P, hs, ncols,
+                           dk, tw);
         return launch_status();
     }
 };

@@ -50,8 +50,11 @@ __device__ __forceinline__ cx<T> apply_mul(int code, cx<T> v, T k) {
     return v;
 }
 
+// Work-group (bx, by) handles column tile bx for the KO outer indices
+// by*KO .. by*KO + KO-1 (consecutive y rows of an x pass stream faster from
+// one work-group).
 template <typename T, int L, int E, int C, int AXIS, int DIR, bool SPLIT, int NIN,
-          typename CODES, int... Rs>
+          typename CODES, int KO, int... Rs>
 __global__ void __launch_bounds__(L / E * C)
     k_multi_strided(MultiPtrs<T, NIN, CODES::n> P, long hs, int ncols, double dk,
                     const cx<T>* __restrict__ tw) {
@@ -64,44 +67,49 @@ __global__ void __launch_bounds__(L / E * C)
     unsigned bx, by;
     xcd_block2(bx, by);
     const int col = bx * C + c;
-    const int o = by;
     const bool ok = col < ncols;
     const bool wok = col < stored_cols<T, C>(ncols, hs);
     const long pstride = (AXIS == 0) ? (long)L * hs : hs;
     const long ostride = (AXIS == 0) ? hs : (long)L * hs;
-    const long base = (long)o * ostride + col;
     // loads: uniform base per element plus one 32-bit per-thread offset
-    const long ubase = (long)o * ostride + (long)bx * C;
     const unsigned toff = (unsigned)(((long)t * pstride + c) * (long)sizeof(cx<T>));
-    cx<T> v[E], u[E];
+#pragma unroll 1
+    for (int i = 0; i < KO; i++) {
+        const int o = (int)by * KO + i;
+        const long base = (long)o * ostride + col;
+        const long ubase = (long)o * ostride + (long)bx * C;
+        cx<T> v[E], u[E];
 #pragma unroll
-    for (int j = 0; j < NOUT; j++) {
-        // per-output copies: LDS addresses and twiddles shared by all outputs
-        // would otherwise stay live across the whole kernel
-        int tl = t, cl = c;
-        const cx<T>* twl = tw;
-        asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
-        if (j == 0 || CODES::src(j) != CODES::src(j > 0 ? j - 1 : 0)) {
-            const char* ib = reinterpret_cast<const char*>(P.in[CODES::src(j)] + ubase);
+        for (int j = 0; j < NOUT; j++) {
+            // per-output copies: LDS addresses and twiddles shared by all
+            // outputs would otherwise stay live across the whole kernel
+            int tl = t, cl = c;
+            const cx<T>* twl = tw;
+            asm volatile("" : "+v"(tl), "+v"(cl), "+s"(twl));
+            if (j == 0 || CODES::src(j) != CODES::src(j > 0 ? j - 1 : 0)) {
+                const char* ib = reinterpret_cast<const char*>(P.in[CODES::src(j)] + ubase);
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const char* eb = ib + (long)TPL * e * pstride * (long)sizeof(cx<T>);
+                    v[e] = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
+                }
+            }
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const char* eb = ib + (long)TPL * e * pstride * (long)sizeof(cx<T>);
-                v[e] = ok ? ld(reinterpret_cast<const cx<T>*>(eb + toff)) : mk<T>(0, 0);
+                const int pos = tl + TPL * e;
+                T k = (T)(freq_index(pos, L) * dk);
+                if (CODES::mul(j) == 3) k *= P.smooth[pos];
+                u[e] = apply_mul(CODES::mul(j), v[e], k);
             }
-        }
+            line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, twl, tl,
+                                                   cl);
+            // padding columns of the last tile hold zeros (zero input, linear map)
+            if (wok) {
+                cx<T>* out = P.out[j];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int pos = tl + TPL * e;
-            T k = (T)(freq_index(pos, L) * dk);
-            if (CODES::mul(j) == 3) k *= P.smooth[pos];
-            u[e] = apply_mul(CODES::mul(j), v[e], k);
-        }
-        line_fft<T, L, E, C, true, DIR, Rs...>(u, lds, SPLIT ? lds : lds + M::SIZE, twl, tl, cl);
-        // padding columns of the last tile hold zeros (zero input, linear map)
-        if (wok) {
-            cx<T>* out = P.out[j];
-#pragma unroll
-            for (int e = 0; e < E; e++) st(out + base + (long)(tl + TPL * e) * pstride, u[e]);
+                for (int e = 0; e < E; e++)
+                    st(out + base + (long)(tl + TPL * e) * pstride, u[e]);
+            }
         }
     }
 }
