@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256)
     int ix[3], iy[3];
     wrap3(cx, N, invN, ix);
     wrap3(cy, N, invN, iy);
-    const unsigned zo = MITOS_PSI_HO + qz;
+    const unsigned zo = MITOS_PSI_SO + qz;
     T Fy_x = 0, Fz_x = 0, Fz_y = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {

@@ -185,6 +185,10 @@ __global__ void __launch_bounds__(256)
 // halo values z = -1 and z = N (copies of z = N-1 and z = 0) beside the row, so
 // the three z taps of a TSC stencil are always three consecutive values.
 #define MITOS_PSI_HO 4
+// The stencil form (the lattice moves read only values at a lattice z, never
+// a z tap) has no halo: value z at offset MITOS_PSI_SO + z = z, so the 64
+// values of a wave's z chunk are two whole 128-byte lines, not three.
+#define MITOS_PSI_SO 0
 
 // FP32 TSC weights: nearest cell c = floor(X + 1/2), u = X - c in [-1/2, 1/2],
 // weights of cells c-1, c, c+1.  Same cells and (up to rounding) the same
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(NT)
     const float invN = 1.0f / N;
     const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
-    const unsigned zo = MITOS_PSI_HO + qz;
+    const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT], qy[PPT];
     unsigned yz[PPT];
     float x[PPT], y[PPT], z[PPT];
@@ -465,7 +469,7 @@ __global__ void __launch_bounds__(64)
     const float invN = 1.0f / N;
     const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
-    const unsigned zo = MITOS_PSI_HO + qz;
+    const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT];
     float x[PPT], y[PPT], z[PPT];
     const unsigned row0 = rowg * PPT;
@@ -725,7 +729,7 @@ __device__ __forceinline__ void moves_s(const T* __restrict__ sx, const T* __res
                                         T& z) {
     const float invN = 1.0f / N;
     const unsigned plane = (unsigned)N * rs;
-    const unsigned yz = (unsigned)qy * rs + MITOS_PSI_HO + qz;
+    const unsigned yz = (unsigned)qy * rs + MITOS_PSI_SO + qz;
     x = (T)(qx * spacing) - sx[(unsigned)qx * plane + yz];
     y = (T)(qy * spacing);
     z = (T)(qz * spacing);
@@ -740,7 +744,7 @@ __device__ __forceinline__ void moves_s(const T* __restrict__ sx, const T* __res
     tsc_axis_near(y * sN, cy, wy);
     unsigned oy[3];
     tap_rows(wrapf(cy, N, invN), N, rs, oy);
-    const unsigned zo = MITOS_PSI_HO + qz;
+    const unsigned zo = MITOS_PSI_SO + qz;
     T d = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++)

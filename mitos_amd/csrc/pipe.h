@@ -374,8 +374,10 @@ struct Pipe {
                                                                 s);
                     }, &sc.ms[TUNE_X_FWD][v]));
                     MITOS_TRY(time_it([&] {
-                        return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(
-                            v, b(0), psi(0), hs, prs(), 1, 0, 0, (T)1, OpNone(), s);
+                        // the displacement grids' layout of the default form
+                        constexpr int HO = sizeof(T) == 4 ? MITOS_PSI_SO : MITOS_PSI_HO;
+                        return zc2r_pass_tuned<T, NN, HO>(v, b(0), psi(0), hs, prs(), 1, 0, 0,
+                                                          (T)1, OpNone(), s);
                     }, &sc.ms[TUNE_Z_C2R][v]));
                     MITOS_TRY(time_it([&] {
                         return zr2c_pass_tuned<T, NN>(v, ld, K, hs, 1, 0, OpNone(), s);
@@ -659,23 +661,22 @@ struct Pipe {
                               OutCodes<MITOS_OUT(0, 1), MITOS_OUT(0, 0), MITOS_OUT(1, 0)>>::go(
                     p7, hs, dk, s);
             }));
-            // 8. z c2r: displacement grids (src/mitos.c:404-416)
+            // 8. z c2r: displacement grids (src/mitos.c:404-416); the
+            // stencil form without the z halo (its moves read no z taps)
             const T cs = (T)(1.0 / boxvol);
-            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(0), psi(0), hs,
-                                                            prs(), 1, 0, 0, cs,
-                                                            OpNone(), s);
-            }));
-            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(1), psi(1), hs,
-                                                            prs(), 1, 0, 0, cs,
-                                                            OpNone(), s);
-            }));
-            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg, [&] {
-                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], b(2), psi(2), hs,
-                                                            prs(), 1, 0, 0, cs,
-                                                            OpIkz{dk}, s);
-            }));
+            auto psi_c2r = [&](const cx<T>* in, T* out, auto op) -> int {
+                if (STENCIL)
+                    return zc2r_pass_tuned<T, NN, MITOS_PSI_SO>(sched.v[TUNE_Z_C2R], in, out, hs,
+                                                                prs(), 1, 0, 0, cs, op, s);
+                return zc2r_pass_tuned<T, NN, MITOS_PSI_HO>(sched.v[TUNE_Z_C2R], in, out, hs,
+                                                            prs(), 1, 0, 0, cs, op, s);
+            };
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg,
+                            [&] { return psi_c2r(b(0), psi(0), OpNone()); }));
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg,
+                            [&] { return psi_c2r(b(1), psi(1), OpNone()); }));
+            MITOS_TRY(timed(s, "displacement_z_c2r", G + Rg,
+                            [&] { return psi_c2r(b(2), psi(2), OpIkz{dk}); }));
             mark(4);
             // 9-10. lattice + sequential TSC displacement + wrap + CIC deposit
             const float spacing = (float)L / NN;

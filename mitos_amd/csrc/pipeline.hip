@@ -273,7 +273,7 @@ static int unstencil_psi(Pipe<T>* P, int N, T* out, hipStream_t s) {
     const double norm = 1.0 / ((double)N * N * N);
     const int masks[3] = {7, 6, 4};
     for (int i = 0; i < 3; i++) {
-        RealRowLoader<T> ld{P->psi(i) + MITOS_PSI_HO, gin.rs, 0};
+        RealRowLoader<T> ld{P->psi(i) + MITOS_PSI_SO, gin.rs, 0};
         MITOS_TRY((r2c_3d<T>(ld, P->K, gin, 1, 0, OpNone(), OpNone(), OpNone(), s)));
         MITOS_TRY((c2r_3d<T>(P->K, P->a(0), out + (size_t)i * N * N * N, gout, 1, 0, 0, (T)1,
                              OpUnstencil{S, masks[i], norm}, OpNone(), OpNone(), s)));
