@@ -87,6 +87,10 @@ __global__ void __launch_bounds__(256)
     const double boxvol = boxlen * boxlen * boxlen;
     const double factor = sqrt(boxvol / 2);
     const int H = N / 2 + 1;
+    // stored z': the modes, and the row padding up to a whole chunk (zeros),
+    // so the last chunk writes whole 128-byte lines
+    const long hz = (long)(H + ZC - 1) / ZC * ZC;
+    const int WZ = (int)(hz < hs ? hz : hs);
     const int nx = freq_index(x, N), ny = freq_index(y, N);
     const long n2xy = (long)nx * nx + (long)ny * ny;
     cx<T>* base = out + blockIdx.y * batch_stride;
@@ -94,7 +98,11 @@ __global__ void __launch_bounds__(256)
         if (ok) {
             for (int dz = 0; dz < ZC; dz++) {
                 const int z = z0 + dz;
-                if (z >= H) break;
+                // past the last mode: zeros for the row padding
+                if (z >= H) {
+                    stage[threadIdx.x * ZC + dz] = mk<T>(0, 0);
+                    continue;
+                }
                 // the DC mode (n^2 = 0, k = 0) takes no draws
                 const long n2 = n2xy + (long)z * z;
                 cx<T> val = mk<T>(0, 0);
@@ -118,7 +126,7 @@ __global__ void __launch_bounds__(256)
             const int idx = i * 256 + threadIdx.x;
             const int r = idx / ZC, dz = idx % ZC;
             const long l = line0 + r;
-            if (l < nrows && z0 + dz < H) st(base + l * hs + z0 + dz, stage[r * ZC + dz]);
+            if (l < nrows && z0 + dz < WZ) st(base + l * hs + z0 + dz, stage[r * ZC + dz]);
         }
         __syncthreads();
     }
