@@ -188,15 +188,16 @@ inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); 
 // kernels: 8 (or 12) values per thread and 64-byte tile rows, so that the
 // input tile, the transformed copy and the radix temporaries stay in VGPRs.
 // Multi-output passes on float lines of 1024: 16 values per thread, so a
-// line is one wave (fewer exchange steps, work-groups of 512 threads); the
-// combine pass keeps the smaller line configuration (MULTI = false).
+// line is one wave (fewer exchange steps), and 128-byte tile rows (whole
+// lines; work-groups of 1024 threads, one per CU); the combine pass keeps the
+// smaller line configuration (MULTI = false).
 template <typename T, int L, bool MULTI = true>
 struct HeavyCfg {
-    typedef typename std::conditional<MULTI && sizeof(T) == 4 && L == 1024, LineCfg<L>,
-                                      LineCfgS<L>>::type CFG;
+    static constexpr bool WIDE = MULTI && sizeof(T) == 4 && L == 1024;
+    typedef typename std::conditional<WIDE, LineCfg<L>, LineCfgS<L>>::type CFG;
     static constexpr int E = CFG::E;
     static constexpr int TPL = L / E;
-    static constexpr int C0 = 64 / (2 * (int)sizeof(T));
+    static constexpr int C0 = (WIDE ? 128 : 64) / (2 * (int)sizeof(T));
     static constexpr int C = (TPL * C0 <= 1024) ? C0 : 1024 / TPL;
 };
 template <typename T, int L, int AXIS, int DIR, int NIN, typename CODES,
