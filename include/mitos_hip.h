@@ -316,7 +316,8 @@ int mitos_ic_destroy(mitos_ic_context *ctx);
  * when MITOS_FFT_SCHEDULE=default disabled the timing); -1 on bad arguments */
 int mitos_ic_schedule(mitos_ic_context *ctx, int kind, float *ms);
 /* mean milliseconds of a plain 3D r2c and c2r of the context's size and
- * precision (real rows padded to N+8, half-complex rows of 128-byte multiples)
+ * precision (real rows one half-complex row long, half-complex rows of
+ * 128-byte multiples)
  * over reps runs, with the autotuned schedules; overwrites the context's
  * grids (run the pipeline again before reading its outputs) */
 int mitos_ic_fft_timing(mitos_ic_context *ctx, int reps, float *r2c_ms, float *c2r_ms);
