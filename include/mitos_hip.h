@@ -16,7 +16,8 @@
  * otherwise (the library never calls exit()).
  *
  * One rank: NX = N, X0 = 0, local_size = N*N*(N/2+1).  Grid sizes with compiled
- * FFT passes: 8, 16, 32, 64, 128, 256, 512, 1024, 1536.
+ * FFT passes: 8, 16, 32, 64, 128, 256, 512, 1024 and 24, 48, 96, 192, 384, 768,
+ * 1536.
  */
 #ifndef MITOS_HIP_H
 #define MITOS_HIP_H
@@ -51,7 +52,8 @@ typedef int MPI_Comm;
 #endif
 #endif
 
-/* error codes (0..2 as src/fft.c:269-277 and src/2lpt.c:69-72) */
+/* error codes: 1 and 2 as fft_apply_kernel_dg returns them (src/fft.c:269-277);
+ * solve2LPT returns 1 for a density not in momentum space (src/2lpt.c:69-72) */
 #define MITOS_OK 0
 #define MITOS_ERR_DIMS 1
 #define MITOS_ERR_NOT_MOMENTUM 2
@@ -131,6 +133,10 @@ int mitos_grid_from_host(struct distributed_grid *dg, const fftw_complex *fbox_h
 /* include/random.h:27-33 (host) */
 uint64_t rand_uint64(rng_state *state);
 rng_state rand_uint64_init(uint64_t seed);
+/* include/random.h:91-94 (host; Box-Muller with the host libm, cosine
+ * branch, two draws per normal, as src/random.c:34-49) */
+double sampleUniform(rng_state *state);
+double sampleNorm(rng_state *state);
 
 /* include/grf.h:30-31 -- the state advances by the 4*(N*N*(N/2+1)-1) draws
  * the reference consumes, so later draws continue the same stream */
@@ -159,10 +165,16 @@ int fft_c2r_dg(struct distributed_grid *dg);
 int fft_apply_kernel_dg(struct distributed_grid *dg_write, const struct distributed_grid *dg_read,
                         void (*compute)(struct kernel *the_kernel), const void *params);
 
-/* Kernel callbacks (include/fft_kernels.h:28-31, 96-121, 198-219).  They
+/* Kernel callbacks (include/fft_kernels.h:28-31, 96-180, 198-219).  They
  * evaluate the kernel on the host when called directly; passed to an
  * fft_apply_kernel* function, their address selects the device kernel.  Any
- * other callback returns MITOS_ERR_UNSUPPORTED (no host fallback). */
+ * other callback -- a caller's own kernel, or the static inline copy of a
+ * reference kernel that include/fft_kernels.h puts in every translation
+ * unit -- is evaluated on the host for every mode (threads over x-slabs, as
+ * the OpenMP loop of src/fft.c:279-291), and the device multiplies the grid
+ * by those values.  That path is host-bound; the replacement
+ * include/fft_kernels.h of this library declares the kernels extern, so a
+ * unit compiled against it reaches the device kernels. */
 void kernel_constant(struct kernel *the_kernel);          /* params: const double* */
 void kernel_inv_poisson(struct kernel *the_kernel);       /* params: unused        */
 void kernel_dx(struct kernel *the_kernel);
@@ -178,6 +190,63 @@ struct mitos_spectrum_params {
 };
 void kernel_sqrt_power_law(struct kernel *the_kernel); /* sqrt(A k^n_s), 0 at k=0 */
 void kernel_bbks_transfer(struct kernel *the_kernel);  /* T_BBKS(k/Gamma), 0 at k=0 */
+
+/* ---- tabulated transfer functions T(k, log tau) ------------------------- */
+/* include/perturb_data.h:25-52 -- same fields, same order (host arrays) */
+struct perturb_data {
+    int k_size;
+    int tau_size;
+    int n_functions;
+    double *delta;     /* k_size * tau_size * n_functions, k fastest */
+    double *k;         /* k_size, increasing */
+    double *log_tau;   /* tau_size, increasing */
+    double *redshift;
+    double *D_growth;
+    double *f_growth;
+    double *H_Hubble;
+    char **titles;     /* n_functions */
+    double *Omega;
+};
+/* include/perturb_spline.h:25-34 */
+struct perturb_spline {
+    const struct perturb_data *ptdat;
+    double *k_acc_table;
+    int k_acc_table_size;
+};
+/* include/fft_kernels.h:123-128: params of kernel_(inv_)transfer_function */
+struct spline_params {
+    const struct perturb_spline *spline;
+    int index_src;
+    int tau_index;
+    double u_tau;
+};
+
+/* src/perturb_spline.c:30-169 (host) */
+int initPerturbSpline(struct perturb_spline *spline, int k_acc_size,
+                      const struct perturb_data *ptdat);
+int cleanPerturbSpline(struct perturb_spline *spline);
+int perturbSplineFindTau(const struct perturb_spline *spline, double log_tau, int *index,
+                         double *u);
+int perturbSplineFindK(const struct perturb_spline *spline, double k, int *index, double *u);
+double perturbSplineInterp(const struct perturb_spline *spline, int k_index, int tau_index,
+                           double u_k, double u_tau, int index_src);
+double perturbSplineInterp0(const struct perturb_spline *spline, double k, double log_tau,
+                            int index_src);
+/* src/titles.c:26-38 */
+int findTitle(char **titles, const char *title, int ntitles);
+
+/* include/fft_kernels.h:131-180: bilinear T(k, log tau) of one tabulated
+ * function, DC -> 0; params: struct spline_params* */
+void kernel_transfer_function(struct kernel *the_kernel);
+void kernel_inv_transfer_function(struct kernel *the_kernel);
+
+/* src/generate_grids.c:28-67: grid = c2r(T(k, log_tau_source) * grf) times
+ * rescale_factor (the growth ratio) when it is not 1; the field file is
+ * written by writeFieldFile_dg when fname is not NULL */
+int generatePerturbationGrid(const struct cosmology *cosmo, const struct perturb_spline *spline,
+                             struct distributed_grid *grf, struct distributed_grid *grid,
+                             const char *transfer_func_title, const char *fname,
+                             const double rescale_factor);
 
 /* include/poisson.h:29-30 */
 int solvePoisson(double *phi, double *f, int N, double boxlen); /* device N^3 arrays */
@@ -200,11 +269,46 @@ void calc_cross_powerspec_float_mpi(long int N, long int X0, long int NX, double
                                     int bins, double *k_in_bins, double *power_in_bins,
                                     long int *obs_in_bins);
 
+/* include/distributed_grid.h:62-72 */
+struct left_right_slice {
+    double *left_slice;  /* device, padded rows of N + 2 values */
+    double *local_slice;
+    double *right_slice;
+    int left_NX;
+    int left_X0;
+    int local_NX;
+    int local_X0;
+    int right_NX;
+    int right_X0;
+};
+/* include/grids_interp.h:36 -- one position (one small device launch; the
+ * batched mitos_displace_velocity_tsc_dg is the form for particle loops);
+ * NaN and mitos_last_error() set on a failure */
+double gridTSC_dg(struct left_right_slice *lrs, double x, double y, double z, double boxlen,
+                  int N);
+
 /* include/grids_interp.h:41-48 -- device arrays */
 int undoCICWindow(fftw_complex *farr, int N, double boxlen);
 int undoTSCWindow(fftw_complex *farr, int N, double boxlen);
 int undoCICWindowFloat(fftwf_complex *farr, int N, double boxlen);
 int undoTSCWindowFloat(fftwf_complex *farr, int N, double boxlen);
+
+/* ---- field files (HDF5: Header/BoxSize, Field/Field doubles) and shrinking */
+/* libhdf5 is loaded at the first call (MITOS_HDF5_LIB, the default search
+ * path, /opt/conda/lib); without it these return MITOS_ERR_UNSUPPORTED. */
+/* include/output.h:31 -- host array N x N x N */
+int writeFieldFile(const double *box, int N, double box_len, const char *fname);
+/* include/input.h:149 -- allocates *box (malloc, host, N^3 unpadded) */
+int readFieldFile(double **box, int *N, double *box_len, const char *fname);
+/* include/output_mpi.h:37 -- the padded device box as an N x N x (N+2)
+ * dataset; returns 1 for a grid in momentum space */
+int writeFieldFile_dg(struct distributed_grid *dg, const char *fname);
+/* include/input_mpi.h:27 -- a padded or unpadded dataset into the device box */
+int readFieldFile_dg(struct distributed_grid *dg, const char *fname);
+/* include/shrink_grids.h:26-29 -- M^3 block averages (M divides N, else 1);
+ * out device M^3; shrinkGrid's in is a device N^3 array */
+int shrinkGrid_dg(double *out, struct distributed_grid *in, int M, int N);
+int shrinkGrid(double *out, const double *in, int M, int N);
 
 /* The FFTW plans the reference builds inline (src/poisson.c:35-36,
  * analyse/src/mitos_read.c:411-413, analyse/src/mitos_dist_read.c:413):
@@ -243,6 +347,29 @@ int mitos_displace_tsc_dg(const struct distributed_grid *psi_x,
                           const struct distributed_grid *psi_y,
                           const struct distributed_grid *psi_z, long long npart, float *X,
                           float *Y, float *Z);
+
+/* src/mitos.c:824-908 then :1072-1077, the whole per-particle loop: the
+ * three sequential displacement moves (psi grids, may all be NULL to skip),
+ * then the velocities v_dir = (float)gridTSC_dg(vel_dir) at the moved, not yet
+ * wrapped position (vel grids and VX/VY/VZ may all be NULL), then the wrap.
+ * Configuration-space grids; device arrays. */
+int mitos_displace_velocity_tsc_dg(const struct distributed_grid *psi_x,
+                                   const struct distributed_grid *psi_y,
+                                   const struct distributed_grid *psi_z,
+                                   const struct distributed_grid *vel_x,
+                                   const struct distributed_grid *vel_y,
+                                   const struct distributed_grid *vel_z, long long npart,
+                                   float *X, float *Y, float *Z, float *VX, float *VY, float *VZ);
+
+/* src/mitos.c:1097-1107: the particle arrays as the IC writer lays them out
+ * before H5Dwrite -- coords and vels npart x 3 doubles (x, y, z per
+ * particle), masses npart doubles, ids npart (index + first_id).  Device
+ * inputs (VX/VY/VZ may be NULL: zero velocities), host outputs (any may be
+ * NULL). */
+int mitos_particles_export(long long npart, const float *X, const float *Y, const float *Z,
+                           const float *VX, const float *VY, const float *VZ, double mass,
+                           long long first_id, double *coords, double *vels, double *masses,
+                           long long *ids);
 
 /* analyse/src/mitos_read.c:278-376: CIC mass assignment of npart particles of
  * equal mass into an unpadded N^3 device grid, then (box-avg)/avg.  The

@@ -77,7 +77,37 @@ class IcResult(ctypes.Structure):
                 ("stage_ms", ctypes.c_float * 8)]
 
 
+class PerturbData(ctypes.Structure):
+    """include/perturb_data.h:25-52 (host arrays)"""
+    _fields_ = [("k_size", ctypes.c_int), ("tau_size", ctypes.c_int),
+                ("n_functions", ctypes.c_int), ("delta", c_double_p), ("k", c_double_p),
+                ("log_tau", c_double_p), ("redshift", c_double_p), ("D_growth", c_double_p),
+                ("f_growth", c_double_p), ("H_Hubble", c_double_p),
+                ("titles", ctypes.POINTER(ctypes.c_char_p)), ("Omega", c_double_p)]
+
+
+class PerturbSpline(ctypes.Structure):
+    """include/perturb_spline.h:25-34"""
+    _fields_ = [("ptdat", ctypes.POINTER(PerturbData)), ("k_acc_table", c_double_p),
+                ("k_acc_table_size", ctypes.c_int)]
+
+
+class SplineParams(ctypes.Structure):
+    """include/fft_kernels.h:123-128"""
+    _fields_ = [("spline", ctypes.POINTER(PerturbSpline)), ("index_src", ctypes.c_int),
+                ("tau_index", ctypes.c_int), ("u_tau", ctypes.c_double)]
+
+
+class LeftRightSlice(ctypes.Structure):
+    """include/distributed_grid.h:62-72 (device slices)"""
+    _fields_ = [("left_slice", vp), ("local_slice", vp), ("right_slice", vp),
+                ("left_NX", ctypes.c_int), ("left_X0", ctypes.c_int),
+                ("local_NX", ctypes.c_int), ("local_X0", ctypes.c_int),
+                ("right_NX", ctypes.c_int), ("right_X0", ctypes.c_int)]
+
+
 KERNEL_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(Kernel))
+DG_P = ctypes.POINTER(DistributedGrid)
 
 # (name, restype, argtypes) of every exported function in include/mitos_hip.h
 SIGNATURES = [
@@ -90,6 +120,8 @@ SIGNATURES = [
     ("mitos_grid_from_host", ctypes.c_int, [ctypes.POINTER(DistributedGrid), vp, vp]),
     ("rand_uint64", ctypes.c_uint64, [ctypes.POINTER(RngState)]),
     ("rand_uint64_init", RngState, [ctypes.c_uint64]),
+    ("sampleUniform", ctypes.c_double, [ctypes.POINTER(RngState)]),
+    ("sampleNorm", ctypes.c_double, [ctypes.POINTER(RngState)]),
     ("generate_complex_grf", ctypes.c_int, [ctypes.POINTER(DistributedGrid),
                                             ctypes.POINTER(RngState)]),
     ("enforce_hermiticity", ctypes.c_int, [ctypes.POINTER(DistributedGrid)]),
@@ -117,6 +149,26 @@ SIGNATURES = [
     ("kernel_undo_Hermite_window", None, [ctypes.POINTER(Kernel)]),
     ("kernel_sqrt_power_law", None, [ctypes.POINTER(Kernel)]),
     ("kernel_bbks_transfer", None, [ctypes.POINTER(Kernel)]),
+    ("initPerturbSpline", ctypes.c_int, [ctypes.POINTER(PerturbSpline), ctypes.c_int,
+                                         ctypes.POINTER(PerturbData)]),
+    ("cleanPerturbSpline", ctypes.c_int, [ctypes.POINTER(PerturbSpline)]),
+    ("perturbSplineFindTau", ctypes.c_int, [ctypes.POINTER(PerturbSpline), ctypes.c_double,
+                                            c_int_p, c_double_p]),
+    ("perturbSplineFindK", ctypes.c_int, [ctypes.POINTER(PerturbSpline), ctypes.c_double,
+                                          c_int_p, c_double_p]),
+    ("perturbSplineInterp", ctypes.c_double, [ctypes.POINTER(PerturbSpline), ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int]),
+    ("perturbSplineInterp0", ctypes.c_double, [ctypes.POINTER(PerturbSpline), ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int]),
+    ("findTitle", ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
+                                 ctypes.c_int]),
+    ("kernel_transfer_function", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_inv_transfer_function", None, [ctypes.POINTER(Kernel)]),
+    ("generatePerturbationGrid", ctypes.c_int, [ctypes.POINTER(Cosmology),
+                                                ctypes.POINTER(PerturbSpline), DG_P, DG_P,
+                                                ctypes.c_char_p, ctypes.c_char_p,
+                                                ctypes.c_double]),
     ("solvePoisson", ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_double]),
     ("solvePoisson_dg", ctypes.c_int, [ctypes.POINTER(DistributedGrid)]),
     ("solve2LPT", ctypes.c_int, [ctypes.POINTER(DistributedGrid)] * 3 + [ctypes.c_double] * 2),
@@ -127,10 +179,20 @@ SIGNATURES = [
     ("calc_cross_powerspec_float_mpi", None, [ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                               ctypes.c_double, vp, vp, ctypes.c_int,
                                               c_double_p, c_double_p, c_long_p]),
+    ("gridTSC_dg", ctypes.c_double, [ctypes.POINTER(LeftRightSlice), ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int]),
     ("undoCICWindow", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
     ("undoTSCWindow", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
     ("undoCICWindowFloat", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
     ("undoTSCWindowFloat", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+    ("writeFieldFile", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_char_p]),
+    ("readFieldFile", ctypes.c_int, [ctypes.POINTER(c_double_p), c_int_p, c_double_p,
+                                     ctypes.c_char_p]),
+    ("writeFieldFile_dg", ctypes.c_int, [DG_P, ctypes.c_char_p]),
+    ("readFieldFile_dg", ctypes.c_int, [DG_P, ctypes.c_char_p]),
+    ("shrinkGrid_dg", ctypes.c_int, [vp, DG_P, ctypes.c_int, ctypes.c_int]),
+    ("shrinkGrid", ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int]),
     ("mitos_fft_r2c", ctypes.c_int, [ctypes.c_int, vp, vp]),
     ("mitos_fft_c2r", ctypes.c_int, [ctypes.c_int, vp, vp]),
     ("mitos_fft_r2c_float", ctypes.c_int, [ctypes.c_int, vp, vp]),
@@ -143,6 +205,10 @@ SIGNATURES = [
                                      vp, vp]),
     ("mitos_displace_tsc_dg", ctypes.c_int, [ctypes.POINTER(DistributedGrid)] * 3 +
      [ctypes.c_longlong, vp, vp, vp]),
+    ("mitos_displace_velocity_tsc_dg", ctypes.c_int, [DG_P] * 6 + [ctypes.c_longlong] +
+     [vp] * 6),
+    ("mitos_particles_export", ctypes.c_int, [ctypes.c_longlong] + [vp] * 6 +
+     [ctypes.c_double, ctypes.c_longlong] + [vp] * 4),
     ("mitos_cic_overdensity", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_longlong,
                                              vp, vp, vp, vp]),
     ("mitos_sigma8_amplitude", ctypes.c_double, [ctypes.c_double] * 4),

@@ -408,6 +408,8 @@ namespace mitos {
 Pipe<float>* ctx_pipe_f(mitos_ic_context* c);
 Pipe<double>* ctx_pipe_d(mitos_ic_context* c);
 hipStream_t ctx_stream(mitos_ic_context* c);
+// orders work issued next on s after the context's last realization
+int ctx_order(mitos_ic_context* c, hipStream_t s);
 // checks cfg against the context and records its bins and box size
 int ctx_use_cfg(mitos_ic_context* c, const mitos_ic_config* cfg);
 }  // namespace mitos
@@ -417,6 +419,7 @@ extern "C" int mitos_ic_noise_forward(mitos_ic_context* c, const struct mitos_ic
     if (!c || !cfg || !noise) return set_last_error(MITOS_ERR_ARG);
     MITOS_TRY(set_last_error(ctx_use_cfg(c, cfg)));
     hipStream_t s = ctx_stream(c);
+    MITOS_TRY(set_last_error(ctx_order(c, s)));
     int rc;
     if (ctx_pipe_f(c))
         rc = ctx_pipe_f(c)->enqueue(*cfg, s, false, nullptr, nullptr, nullptr,
@@ -437,6 +440,7 @@ extern "C" int mitos_ic_noise_backward(mitos_ic_context* c, const struct mitos_i
         return set_last_error(MITOS_ERR_ARG);
     MITOS_TRY(set_last_error(ctx_use_cfg(c, cfg)));
     hipStream_t s = ctx_stream(c);
+    MITOS_TRY(set_last_error(ctx_order(c, s)));
     int rc;
     if (ctx_pipe_f(c))
         rc = backward<float>(ctx_pipe_f(c), *cfg, (const cx<float>*)noise, dL_dP,

@@ -3,6 +3,10 @@
 //  include/2lpt.h:27-29, include/grids_interp.h:41-48).
 #include <string.h>
 
+#include <algorithm>
+#include <thread>
+#include <vector>
+
 #include "../../include/mitos_hip.h"
 #include "kspace.h"
 
@@ -65,12 +69,117 @@ int kspec_from_callback(void (*compute)(struct kernel*), const void* params, int
     return MITOS_OK;
 }
 
+// kernel_(inv_)transfer_function: the k vector and the selected function's
+// two time rows go to the device, the modes are evaluated there
+template <typename TW, typename TR>
+int apply_transfer(cx<TW>* w, const cx<TR>* r, int N, long x0, long nx, double boxlen,
+                   const struct spline_params* sp, int invert) {
+    if (!sp || !sp->spline || !sp->spline->ptdat) return MITOS_ERR_ARG;
+    const struct perturb_data* pt = sp->spline->ptdat;
+    const int ks = pt->k_size, ts = pt->tau_size;
+    if (ks < 2 || ts < 2 || !pt->k || !pt->delta || sp->index_src < 0 ||
+        sp->index_src >= pt->n_functions || sp->tau_index < 0 || sp->tau_index > ts - 2)
+        return MITOS_ERR_ARG;
+    double* d = (double*)workspace(5, sizeof(double) * 3 * ks);
+    if (!d) return MITOS_ERR_ALLOC;
+    const double* rows = pt->delta + (size_t)sp->index_src * ks * ts + (size_t)ks * sp->tau_index;
+    MITOS_HIP_CHECK(copy_sync(d, pt->k, sizeof(double) * ks, hipMemcpyHostToDevice));
+    MITOS_HIP_CHECK(copy_sync(d + ks, rows, sizeof(double) * 2 * ks, hipMemcpyHostToDevice));
+    TransferTab t{d, d + ks, d + 2 * ks, ks, sp->u_tau, invert};
+    const long n = nx * N * (N / 2 + 1);
+    hipLaunchKernelGGL((k_apply_transfer<TW, TR>), dim3(nblk(n)), dim3(256), 0, api_stream(), w,
+                       r, N, (long)(N / 2 + 1), x0, nx, boxlen, t);
+    MITOS_TRY(launch_status());
+    return api_sync();
+}
+
+// Any other callback: evaluated on the host for every mode with the
+// reference's protocol (src/fft.c:279-291: {kx, ky, kz, k, 0, params} in,
+// kern out), a chunk of x-slabs at a time over host threads (the callback
+// runs concurrently, as under the reference's OpenMP loop), copied to the
+// device through two pinned buffers, and multiplied in there.
+template <typename TW, typename TR>
+int apply_kernel_host(cx<TW>* w, const cx<TR>* r, int N, long x0, long nx, double boxlen,
+                      void (*compute)(struct kernel*), const void* params) {
+    const long H = N / 2 + 1;
+    const long plane = (long)N * H;
+    const long bytes_slab = plane * (long)sizeof(cx<double>);
+    const long per = std::max(1L, (32L << 20) / bytes_slab);  // slabs per chunk
+    const size_t cbytes = (size_t)(per * bytes_slab);
+    cx<double>* hbuf[2] = {nullptr, nullptr};
+    cx<double>* dbuf = (cx<double>*)workspace(4, 2 * cbytes);
+    if (!dbuf) return MITOS_ERR_ALLOC;
+    hipEvent_t used[2] = {nullptr, nullptr};
+    int rc = MITOS_OK;
+    for (int b = 0; b < 2 && rc == MITOS_OK; b++) {
+        if (hipHostMalloc((void**)&hbuf[b], cbytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&used[b], hipEventDisableTiming) != hipSuccess)
+            rc = MITOS_ERR_ALLOC;
+    }
+    const double dk = 2 * M_PI / boxlen;
+    const int nthreads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    for (long c = 0, it = 0; c < nx && rc == MITOS_OK; c += per, it++) {
+        const long ns = std::min(per, nx - c);
+        const int b = (int)(it & 1);
+        if (it >= 2 && hipEventSynchronize(used[b]) != hipSuccess) {
+            rc = MITOS_ERR_DEVICE;
+            break;
+        }
+        cx<double>* hb = hbuf[b];
+        auto worker = [&](int tid) {
+            for (long row = tid; row < ns * N; row += nthreads) {
+                const int x = (int)(x0 + c + row / N), y = (int)(row % N);
+                for (int z = 0; z < H; z++) {
+                    struct kernel kk;
+                    wavevec(x, y, z, N, dk, kk.kx, kk.ky, kk.kz, kk.k);
+                    double* kp = kern_parts(&kk);
+                    kp[0] = 0;
+                    kp[1] = 0;
+                    kk.params = params;
+                    compute(&kk);
+                    hb[row * H + z] = mk<double>(kp[0], kp[1]);
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthreads; t++) pool.emplace_back(worker, t);
+        worker(0);
+        for (auto& th : pool) th.join();
+        cx<double>* db = dbuf + (size_t)b * per * plane;
+        if (hipMemcpyAsync(db, hb, (size_t)(ns * bytes_slab), hipMemcpyHostToDevice,
+                           api_stream()) != hipSuccess) {
+            rc = MITOS_ERR_DEVICE;
+            break;
+        }
+        const long off = c * N * H;
+        hipLaunchKernelGGL((k_apply_mult<TW, TR>), dim3(nblk(ns * plane)), dim3(256), 0,
+                           api_stream(), w + off, r + off, db, N, H, ns);
+        rc = launch_status();
+        if (rc == MITOS_OK && hipEventRecord(used[b], api_stream()) != hipSuccess)
+            rc = MITOS_ERR_DEVICE;
+    }
+    if (rc == MITOS_OK) rc = api_sync();
+    else (void)api_sync();
+    for (int b = 0; b < 2; b++) {
+        if (hbuf[b]) hipHostFree(hbuf[b]);
+        if (used[b]) hipEventDestroy(used[b]);
+    }
+    return rc;
+}
+
 template <typename TW, typename TR>
 int apply_kernel(cx<TW>* w, const cx<TR>* r, int N, long x0, long nx, double boxlen,
                  void (*compute)(struct kernel*), const void* params) {
-    if (!w || !r || N < 2) return MITOS_ERR_ARG;
+    if (!w || !r || N < 2 || !compute) return MITOS_ERR_ARG;
+    if (compute == kernel_transfer_function || compute == kernel_inv_transfer_function)
+        return apply_transfer<TW, TR>(w, r, N, x0, nx, boxlen,
+                                      (const struct spline_params*)params,
+                                      compute == kernel_inv_transfer_function);
     KSpec ks;
-    MITOS_TRY(kspec_from_callback(compute, params, N, boxlen, &ks));
+    const int kr = kspec_from_callback(compute, params, N, boxlen, &ks);
+    if (kr == MITOS_ERR_UNSUPPORTED)
+        return apply_kernel_host<TW, TR>(w, r, N, x0, nx, boxlen, compute, params);
+    MITOS_TRY(kr);
     const long n = nx * N * (N / 2 + 1);
     hipLaunchKernelGGL((k_apply_kernel<TW, TR>), dim3(nblk(n)), dim3(256), 0, api_stream(), w,
                        r, N, (long)(N / 2 + 1), x0, nx, boxlen, ks);
@@ -357,7 +466,9 @@ extern "C" int solve2LPT(struct distributed_grid* potential, struct distributed_
     if (potential->box == density->box || workspace_grid->box == density->box ||
         potential->box == workspace_grid->box)
         return set_last_error(MITOS_ERR_ARG);
-    if (density->momentum_space != 1) return set_last_error(MITOS_ERR_NOT_MOMENTUM);
+    // src/2lpt.c:69-72 returns 1 here (fft_apply_kernel_dg returns 2 for the
+    // same condition, src/fft.c:274)
+    if (density->momentum_space != 1) return set_last_error(1);
 
     void (*const derivatives[3])(struct kernel*) = {kernel_dx, kernel_dy, kernel_dz};
     const int index_a[] = {0, 0, 0, 1, 1, 2};

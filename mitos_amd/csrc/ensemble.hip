@@ -41,6 +41,10 @@ struct mitos_ensemble {
     long long* obs = nullptr;  // device [n][bmax]
     hipStream_t root = nullptr;
     hipEvent_t fork = nullptr;
+    // recorded on the launch stream right after each graph launch: the next
+    // launch and the P(k) fetch wait for it
+    hipEvent_t launched = nullptr;
+    hipStream_t launch_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -78,6 +82,7 @@ struct mitos_ensemble {
         hipFree(pk);
         hipFree(obs);
         if (fork) hipEventDestroy(fork);
+        if (launched) hipEventDestroy(launched);
         if (root) hipStreamDestroy(root);
     }
 };
@@ -103,6 +108,7 @@ static int ensemble_build(mitos_ensemble* e) {
     }
     MITOS_HIP_CHECK(hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking));
     MITOS_HIP_CHECK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+    MITOS_HIP_CHECK(hipEventCreateWithFlags(&e->launched, hipEventDisableTiming));
     for (auto& g : e->groups) {
         MITOS_HIP_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
         MITOS_HIP_CHECK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
@@ -167,13 +173,19 @@ extern "C" int mitos_ensemble_create(const struct mitos_ic_config* cfgs, int n,
 extern "C" int mitos_ensemble_enqueue(mitos_ensemble* e, void* stream) {
     if (!e) return set_last_error(MITOS_ERR_ARG);
     hipStream_t s = stream ? (hipStream_t)stream : e->root;
+    // the graph reuses one set of grids per group: a launch on another stream
+    // waits for the previous one on the device
+    if (e->launch_stream && e->launch_stream != s)
+        MITOS_HIP_CHECK(hipStreamWaitEvent(s, e->launched, 0));
     MITOS_HIP_CHECK(hipGraphLaunch(e->exec, s));
+    MITOS_HIP_CHECK(hipEventRecord(e->launched, s));
+    e->launch_stream = s;
     return MITOS_OK;
 }
 
 extern "C" int mitos_ensemble_run(mitos_ensemble* e) {
     MITOS_TRY(mitos_ensemble_enqueue(e, nullptr));
-    MITOS_HIP_CHECK(hipStreamSynchronize(e->root));
+    MITOS_HIP_CHECK(hipEventSynchronize(e->launched));
     MITOS_HIP_CHECK(hipGetLastError());
     return MITOS_OK;
 }
@@ -183,6 +195,8 @@ extern "C" int mitos_ensemble_fetch_pk(mitos_ensemble* e, int i, struct mitos_ic
     const int bins = e->cfg[i].bins;
     std::vector<double> sums(2 * bins);
     std::vector<long long> obs(bins);
+    // the last launch, on whichever stream it was issued
+    if (e->launch_stream) MITOS_HIP_CHECK(hipEventSynchronize(e->launched));
     MITOS_HIP_CHECK(hipStreamSynchronize(e->root));
     MITOS_HIP_CHECK(hipMemcpy(sums.data(), e->pk + (size_t)i * 2 * e->bmax,
                               sizeof(double) * 2 * bins, hipMemcpyDeviceToHost));

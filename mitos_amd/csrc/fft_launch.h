@@ -11,17 +11,23 @@
 namespace mitos {
 
 // Grid sizes with compiled FFT passes (line length N for the x/y passes, N/2
-// for the z passes).
+// for the z passes): powers of two from 8 to 1024 and 3*2^k from 24 to 1536.
 template <typename F>
 int with_size(int N, F&& f) {
     switch (N) {
         case 8: return f.template operator()<8>();
         case 16: return f.template operator()<16>();
+        case 24: return f.template operator()<24>();
         case 32: return f.template operator()<32>();
+        case 48: return f.template operator()<48>();
         case 64: return f.template operator()<64>();
+        case 96: return f.template operator()<96>();
         case 128: return f.template operator()<128>();
+        case 192: return f.template operator()<192>();
         case 256: return f.template operator()<256>();
+        case 384: return f.template operator()<384>();
         case 512: return f.template operator()<512>();
+        case 768: return f.template operator()<768>();
         case 1024: return f.template operator()<1024>();
         case 1536: return f.template operator()<1536>();
         default: return MITOS_ERR_UNSUPPORTED;
@@ -29,8 +35,13 @@ int with_size(int N, F&& f) {
 }
 
 inline bool size_supported(int N) {
-    return N == 8 || N == 16 || N == 32 || N == 64 || N == 128 || N == 256 || N == 512 ||
-           N == 1024 || N == 1536;
+    switch (N) {
+        case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
+        case 256: case 384: case 512: case 768: case 1024: case 1536:
+            return true;
+        default:
+            return false;
+    }
 }
 
 // Device twiddle table tw[k] = exp(-2 pi i k / L), k < L, one per (device, L, type).

@@ -25,13 +25,21 @@ struct Radices {};
 // line configuration per length: E values per thread and the stage radices
 template <int L>
 struct LineCfg;
+// (lengths 3*2^k hold 12 values per thread: radices 4, 2 and a final 3, so
+// every line of the 3*2^k grids has a power-of-two thread count)
 template <> struct LineCfg<4>    { static constexpr int E = 4;  typedef Radices<4> R; };
 template <> struct LineCfg<8>    { static constexpr int E = 8;  typedef Radices<8> R; };
+template <> struct LineCfg<12>   { static constexpr int E = 12; typedef Radices<4, 3> R; };
 template <> struct LineCfg<16>   { static constexpr int E = 16; typedef Radices<16> R; };
+template <> struct LineCfg<24>   { static constexpr int E = 12; typedef Radices<4, 2, 3> R; };
 template <> struct LineCfg<32>   { static constexpr int E = 8;  typedef Radices<8, 4> R; };
+template <> struct LineCfg<48>   { static constexpr int E = 12; typedef Radices<4, 4, 3> R; };
 template <> struct LineCfg<64>   { static constexpr int E = 8;  typedef Radices<8, 8> R; };
+template <> struct LineCfg<96>   { static constexpr int E = 12; typedef Radices<4, 4, 2, 3> R; };
 template <> struct LineCfg<128>  { static constexpr int E = 16; typedef Radices<16, 8> R; };
+template <> struct LineCfg<192>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 3> R; };
 template <> struct LineCfg<256>  { static constexpr int E = 16; typedef Radices<16, 16> R; };
+template <> struct LineCfg<384>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 2, 3> R; };
 template <> struct LineCfg<512>  { static constexpr int E = 8;  typedef Radices<8, 8, 8> R; };
 template <> struct LineCfg<768>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 4, 3> R; };
 template <> struct LineCfg<1024> { static constexpr int E = 16; typedef Radices<16, 16, 4> R; };
@@ -43,11 +51,17 @@ template <int L>
 struct LineCfgS;
 template <> struct LineCfgS<4>    { static constexpr int E = 4;  typedef Radices<4> R; };
 template <> struct LineCfgS<8>    { static constexpr int E = 8;  typedef Radices<8> R; };
+template <> struct LineCfgS<12>   { static constexpr int E = 12; typedef Radices<4, 3> R; };
 template <> struct LineCfgS<16>   { static constexpr int E = 4;  typedef Radices<4, 4> R; };
+template <> struct LineCfgS<24>   { static constexpr int E = 12; typedef Radices<4, 2, 3> R; };
 template <> struct LineCfgS<32>   { static constexpr int E = 8;  typedef Radices<8, 4> R; };
+template <> struct LineCfgS<48>   { static constexpr int E = 12; typedef Radices<4, 4, 3> R; };
 template <> struct LineCfgS<64>   { static constexpr int E = 8;  typedef Radices<8, 8> R; };
+template <> struct LineCfgS<96>   { static constexpr int E = 12; typedef Radices<4, 4, 2, 3> R; };
 template <> struct LineCfgS<128>  { static constexpr int E = 8;  typedef Radices<8, 8, 2> R; };
+template <> struct LineCfgS<192>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 3> R; };
 template <> struct LineCfgS<256>  { static constexpr int E = 8;  typedef Radices<8, 8, 4> R; };
+template <> struct LineCfgS<384>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 2, 3> R; };
 template <> struct LineCfgS<512>  { static constexpr int E = 8;  typedef Radices<8, 8, 8> R; };
 template <> struct LineCfgS<768>  { static constexpr int E = 12; typedef Radices<4, 4, 4, 4, 3> R; };
 template <> struct LineCfgS<1024> { static constexpr int E = 8;  typedef Radices<8, 8, 8, 2> R; };

@@ -81,6 +81,90 @@ __global__ void k_apply_kernel(cx<TW>* w, const cx<TR>* r, int N, long hs, long 
 }
 
 
+// kernel_(inv_)transfer_function (include/fft_kernels.h:131-180) on the
+// device: one tabulated function at two neighbouring times, tau_index and
+// tau_index + 1 (the rows lo and hi), interpolated bilinearly as
+// perturbSplineInterp does (src/perturb_spline.c:141-160).
+struct TransferTab {
+    const double* k;   // k_size wavenumbers, increasing
+    const double* lo;  // T(k_j) at tau_index
+    const double* hi;  // T(k_j) at tau_index + 1
+    int k_size;
+    double u_tau;
+    int invert;        // 1: kernel_inv_transfer_function (1/T)
+};
+
+// perturbSplineFindK (src/perturb_spline.c:100-138) returns the first i from
+// its search start with k_i <= k <= k_{i+1}, i.e. the largest i with k_i < k
+// (0 when k = k_0), and i = k_size - 2, u = 1 above the table.  The same
+// interval by bisection; below k_0 (where the reference's search has no
+// defined result) the first interval extrapolates linearly.
+__host__ __device__ __forceinline__ double transfer_value(const TransferTab& t, double k) {
+#pragma clang fp contract(off)
+    const int n = t.k_size;
+    int i;
+    double u;
+    if (k > t.k[n - 1]) {
+        i = n - 2;
+        u = 1.0;
+    } else {
+        int lo = 0, hi = n - 2;  // the answer lies in [lo, hi]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (t.k[mid] < k)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        i = lo;
+        const double left = t.k[i], right = t.k[i + 1];
+        u = (k - left) / (right - left);
+    }
+    const double T11 = t.lo[i], T21 = t.lo[i + 1], T12 = t.hi[i], T22 = t.hi[i + 1];
+    const double v = (1 - t.u_tau) * ((1 - u) * T11 + u * T21) +
+                     t.u_tau * ((1 - u) * T12 + u * T22);
+    return t.invert ? 1.0 / v : v;
+}
+
+template <typename TW, typename TR>
+__global__ void k_apply_transfer(cx<TW>* w, const cx<TR>* r, int N, long hs, long x0, long nx,
+                                 double boxlen, TransferTab t) {
+#pragma clang fp contract(off)
+    const long H = N / 2 + 1;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * N * H) return;
+    const int z = (int)(i % H);
+    const long xy = i / H;
+    const int y = (int)(xy % N);
+    const int x = (int)(x0 + xy / N);
+    const double dk = 2 * M_PI / boxlen;
+    double kx, ky, kz, k;
+    wavevec(x, y, z, N, dk, kx, ky, kz, k);
+    const double kv = (k == 0) ? 0.0 : transfer_value(t, k);  // DC ignored
+    const long id = ((long)(x - x0) * N + y) * hs + z;
+    const cx<TR> a = r[id];
+    const double re = (double)a.re, im = (double)a.im;
+    w[id] = mk<TW>((TW)(re * kv - im * 0.0), (TW)(re * 0.0 + im * kv));
+}
+
+// write = read * m over nx x-slabs, m the kernel values evaluated on the host
+// (dense N x N x (N/2+1) per slab)
+template <typename TW, typename TR>
+__global__ void k_apply_mult(cx<TW>* w, const cx<TR>* r, const cx<double>* __restrict__ m,
+                             int N, long hs, long nx) {
+#pragma clang fp contract(off)
+    const long H = N / 2 + 1;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * N * H) return;
+    const long z = i % H;
+    const long xy = i / H;
+    const long id = xy * hs + z;
+    const cx<TR> a = r[id];
+    const cx<double> kv = m[i];
+    const double re = (double)a.re, im = (double)a.im;
+    w[id] = mk<TW>((TW)(re * kv.re - im * kv.im), (TW)(re * kv.im + im * kv.re));
+}
+
 // elementwise complex scaling (normalisations)
 template <typename T>
 __global__ void k_scale_complex(cx<T>* a, long n, double s) {

@@ -215,9 +215,24 @@ __device__ __forceinline__ void tap_rows(int cb, int N, unsigned stride, unsigne
 // TSC interpolation from a halo-padded grid.  REF = true: the weights and the
 // 27-term sum of gridTSC_dg (src/grids_interp.c:267-309) with every operation
 // rounded as there.  REF = false (float): separable weights and fused sums.
+// Nearest cell of a position in cell units given in double, and the three
+// TSC weights in M from the offset u = X - c (exact in double, then rounded
+// once), so a float gather weights its taps at the position the reference
+// forms in double (x*N/boxlen, src/grids_interp.c:269-271).
+template <typename M>
+__device__ __forceinline__ void tsc_axis_near_pos(double X, int& c, M w[3]) {
+    const double cf = floor(X + 0.5);
+    const M u = (M)(X - cf);
+    c = (int)cf;
+    const M a = (M)0.5 - u, b = (M)0.5 + u;
+    w[0] = (M)0.5 * (a * a);
+    w[1] = (M)0.75 - u * u;
+    w[2] = (M)0.5 * (b * b);
+}
+
 template <typename T, typename M, bool REF>
 __device__ __forceinline__ M tsc_halo(const T* __restrict__ box, unsigned rs, int N, float invN,
-                                      M X, M Y, M Z) {
+                                      double X, double Y, double Z) {
 #pragma clang fp contract(off)
     int cx, cy, cz;
     M wx[3], wy[3], wz[3];
@@ -229,9 +244,9 @@ __device__ __forceinline__ M tsc_halo(const T* __restrict__ box, unsigned rs, in
         cy++;
         cz++;
     } else {
-        tsc_axis_near(X, cx, wx);
-        tsc_axis_near(Y, cy, wy);
-        tsc_axis_near(Z, cz, wz);
+        tsc_axis_near_pos(X, cx, wx);
+        tsc_axis_near_pos(Y, cy, wy);
+        tsc_axis_near_pos(Z, cz, wz);
     }
     const int xb = wrapf(cx, N, invN), yb = wrapf(cy, N, invN), zb = wrapf(cz, N, invN);
     unsigned ox[3], oy[3];
@@ -269,7 +284,7 @@ template <typename T, typename M, int NT>
 __global__ void __launch_bounds__(NT)
     k_displace_lattice(const T* __restrict__ px, const T* __restrict__ py,
                        const T* __restrict__ pz, unsigned rs, int N, int nch, double boxlen,
-                       float spacing, float cells_per_len, float* __restrict__ X,
+                       float spacing, float* __restrict__ X,
                        float* __restrict__ Y,
                        float* __restrict__ Z) {
 #pragma clang fp contract(off)
@@ -293,12 +308,17 @@ __global__ void __launch_bounds__(NT)
                                       (double)zz * N / bl);
         zz = (float)((double)zz - d);
     } else {
-        const float s = cells_per_len;
-        float d = tsc_halo<T, float, false>(px, rs, N, invN, x * s, y * s, zz * s);
+        // cell coordinates as the reference forms them (double x*N/boxlen);
+        // weights, sums and moves in float
+        const double bl = boxlen;
+        float d = tsc_halo<T, float, false>(px, rs, N, invN, (double)x * N / bl,
+                                            (double)y * N / bl, (double)zz * N / bl);
         x = x - d;
-        d = tsc_halo<T, float, false>(py, rs, N, invN, x * s, y * s, zz * s);
+        d = tsc_halo<T, float, false>(py, rs, N, invN, (double)x * N / bl, (double)y * N / bl,
+                                      (double)zz * N / bl);
         y = y - d;
-        d = tsc_halo<T, float, false>(pz, rs, N, invN, x * s, y * s, zz * s);
+        d = tsc_halo<T, float, false>(pz, rs, N, invN, (double)x * N / bl, (double)y * N / bl,
+                                      (double)zz * N / bl);
         zz = zz - d;
     }
     const long long p = (long long)row * N + z;
@@ -358,7 +378,6 @@ __global__ void __launch_bounds__(NT)
     const int qz = (int)(bid - rowg * (unsigned)nch) * NT + threadIdx.x;
     if (qz >= N) return;
     const float invN = 1.0f / N;
-    const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
     const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT], qy[PPT];
@@ -392,7 +411,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         int cx;
-        tsc_axis_near((T)x[j] * Nf / Lf, cx, wx[j]);
+        tsc_axis_near_pos<T>((double)x[j] * N / boxlen, cx, wx[j]);
         tap_rows(wrapf(cx, N, invN), N, plane, ox[j]);
 #pragma unroll
         for (int a = 0; a < 3; a++) syv[j][a] = sy[ox[j][a] + yz[j]];
@@ -407,7 +426,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         int cy;
-        tsc_axis_near((T)y[j] * Nf / Lf, cy, wy[j]);
+        tsc_axis_near_pos<T>((double)y[j] * N / boxlen, cy, wy[j]);
         unsigned oy[3];
         tap_rows(wrapf(cy, N, invN), N, rs, oy);
 #pragma unroll
@@ -467,7 +486,6 @@ __global__ void __launch_bounds__(64)
     const int lane = threadIdx.x;
     const int qz = (int)(bid - rowg * (unsigned)nch) * 64 + lane;
     const float invN = 1.0f / N;
-    const T Nf = (T)N;
     const unsigned plane = (unsigned)N * rs;
     const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT];
@@ -496,7 +514,7 @@ __global__ void __launch_bounds__(64)
     int lo = 1 << 30, hi = -(1 << 30);
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        tsc_axis_near((T)x[j] * Nf / Lf, cx[j], wx[j]);
+        tsc_axis_near_pos<T>((double)x[j] * N / boxlen, cx[j], wx[j]);
         lo = cx[j] < lo ? cx[j] : lo;
         hi = cx[j] > hi ? cx[j] : hi;
     }
@@ -535,7 +553,7 @@ __global__ void __launch_bounds__(64)
     int ylo = 1 << 30, yhi = -(1 << 30);
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        tsc_axis_near((T)y[j] * Nf / Lf, cy[j], wy[j]);
+        tsc_axis_near_pos<T>((double)y[j] * N / boxlen, cy[j], wy[j]);
         ylo = cy[j] < ylo ? cy[j] : ylo;
         yhi = cy[j] > yhi ? cy[j] : yhi;
     }

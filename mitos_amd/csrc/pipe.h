@@ -101,6 +101,21 @@ inline const V* smooth_table(int N) {
     return d;
 }
 
+// True when every lattice coordinate q*((float)L/N) (float, src/particle.c:
+// 102-109) sits exactly on grid node q in the reference's cell units
+// (double x*N/boxlen, src/grids_interp.c:269-271).  Then the TSC window of an
+// unmoved coordinate is exactly (1/8, 3/4, 1/8) and the lattice stencils
+// reproduce gridTSC_dg; otherwise (e.g. 2000 Mpc/h over 1536 cells, a spacing
+// float rounds) the moves gather 27 taps at the actual positions.
+inline bool lattice_on_nodes(int N, double L) {
+    const float spacing = (float)L / N;
+    for (int q = 0; q < N; q++) {
+        const float x = (float)q * spacing;
+        if ((double)x * N / L != (double)q) return false;
+    }
+    return true;
+}
+
 inline double amplitude_cached(double n_s, double gamma, double sigma8) {
     static std::mutex mu;
     static std::map<std::tuple<double, double, double>, double> cache;
@@ -535,7 +550,9 @@ struct Pipe {
                 const double* amp_tab = nullptr, double* pk_dst = nullptr,
                 long long* obs_dst = nullptr, const cx<T>* noise_in = nullptr,
                 int stencil = -1) {
-        const bool STENCIL = stencil < 0 ? sizeof(T) == 4 : stencil != 0;
+        // the stencil form is exact only for particles on grid nodes
+        const bool STENCIL =
+            stencil < 0 ? (sizeof(T) == 4 && lattice_on_nodes(N, cfg.boxlen)) : stencil != 0;
         psi_stencilled = STENCIL;
         const double L = cfg.boxlen;
         const double dk = 2 * M_PI / L;
@@ -726,7 +743,7 @@ struct Pipe {
                     } else {
                         hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,
                                            psi(0), psi(1), psi(2), rs, NN, nch, L, spacing,
-                                           (float)(NN / L), X, Y, Z);
+                                           X, Y, Z);
                     }
                     return launch_status();
                 }));

@@ -19,6 +19,7 @@ struct mitos_ic_context {
     double boxlen;
     hipStream_t stream;
     hipStream_t last = nullptr;  // stream of the last enqueued realization
+    hipEvent_t order = nullptr;  // orders a new realization after the last one
     bool own_stream;
     bool timing;
     Pipe<float>* pf;
@@ -45,6 +46,11 @@ extern "C" int mitos_ic_create(const struct mitos_ic_config* cfg, mitos_ic_conte
     c->pf = nullptr;
     c->pd = nullptr;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_last_error(MITOS_ERR_DEVICE);
+    }
+    if (hipEventCreateWithFlags(&c->order, hipEventDisableTiming) != hipSuccess) {
+        hipStreamDestroy(c->stream);
         delete c;
         return set_last_error(MITOS_ERR_DEVICE);
     }
@@ -77,6 +83,7 @@ extern "C" int mitos_ic_destroy(mitos_ic_context* c) {
         delete c->pd;
     }
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    if (c->order) hipEventDestroy(c->order);
     delete c;
     return MITOS_OK;
 }
@@ -85,6 +92,19 @@ extern "C" int mitos_ic_destroy(mitos_ic_context* c) {
 // context's own stream reads the pipeline buffers
 static int wait_caller_stream(mitos_ic_context* c) {
     if (c->last && c->last != c->stream) MITOS_HIP_CHECK(hipStreamSynchronize(c->last));
+    return MITOS_OK;
+}
+
+// Work about to be issued on stream s reuses the pipeline buffers (K, A, B,
+// X/Y/Z, P(k) sums) of the last realization, which may still run on another
+// stream: s waits for it on the device (an event, so the order also holds
+// inside a graph capture of s), and s becomes the last stream.
+static int order_after_last(mitos_ic_context* c, hipStream_t s) {
+    if (c->last && c->last != s) {
+        MITOS_HIP_CHECK(hipEventRecord(c->order, c->last));
+        MITOS_HIP_CHECK(hipStreamWaitEvent(s, c->order, 0));
+    }
+    c->last = s;
     return MITOS_OK;
 }
 
@@ -99,6 +119,7 @@ namespace mitos {
 Pipe<float>* ctx_pipe_f(mitos_ic_context* c) { return c->pf; }
 Pipe<double>* ctx_pipe_d(mitos_ic_context* c) { return c->pd; }
 hipStream_t ctx_stream(mitos_ic_context* c) { return c->stream; }
+int ctx_order(mitos_ic_context* c, hipStream_t s) { return order_after_last(c, s); }
 int ctx_use_cfg(mitos_ic_context* c, const mitos_ic_config* cfg) {
     MITOS_TRY(check_cfg(c, cfg));
     c->bins = cfg->bins;
@@ -113,7 +134,7 @@ extern "C" int mitos_ic_enqueue(mitos_ic_context* c, const struct mitos_ic_confi
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     c->bins = cfg->bins;
     c->boxlen = cfg->boxlen;
-    c->last = s;
+    MITOS_TRY(set_last_error(order_after_last(c, s)));
     int rc = c->pf ? c->pf->enqueue(*cfg, s, c->timing && !stream)
                    : c->pd->enqueue(*cfg, s, c->timing && !stream);
     return set_last_error(rc);
@@ -170,6 +191,7 @@ extern "C" int mitos_ic_schedule(mitos_ic_context* c, int kind, float* ms) {
 extern "C" int mitos_ic_fft_timing(mitos_ic_context* c, int reps, float* r2c_ms,
                                    float* c2r_ms) {
     if (!c || reps < 1 || !r2c_ms || !c2r_ms) return set_last_error(MITOS_ERR_ARG);
+    MITOS_TRY(set_last_error(order_after_last(c, c->stream)));
     const int rc = c->pf ? c->pf->fft_timing(c->stream, reps, r2c_ms, c2r_ms)
                          : c->pd->fft_timing(c->stream, reps, r2c_ms, c2r_ms);
     return set_last_error(rc);

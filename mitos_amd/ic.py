@@ -211,6 +211,17 @@ class IC:
         cfg = cfg or self.cfg
         check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
 
+    def fetch_pk(self) -> dict:
+        """P(k) of the last enqueued realization (waits for its stream)."""
+        bins = self.cfg.bins
+        k = np.empty(bins)
+        P = np.empty(bins)
+        obs = np.empty(bins, np.int64)
+        res = _lib.IcResult(k.ctypes.data_as(_lib.c_double_p), P.ctypes.data_as(_lib.c_double_p),
+                            obs.ctypes.data_as(_lib.c_ll_p), 0.0)
+        check(lib().mitos_ic_fetch_pk(self.h, ctypes.byref(res)), "mitos_ic_fetch_pk")
+        return {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude}
+
     def noise_forward(self, cfg: ICConfig, noise) -> dict:
         """P(k) of the pipeline run from `noise` (torch CUDA complex tensor of
         shape (N, N, N/2+1), complex64 for FP32 contexts, complex128 for FP64)."""

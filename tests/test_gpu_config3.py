@@ -1,0 +1,193 @@
+"""GPU, the FP32 precision variant at 1024^3 and 1536^3 in a 2000 Mpc/h box.
+
+  * 1024^3: the FP32 fused pipeline against the FP64 fused pipeline of the
+    same realization (seed 42, 128 bins): per-bin P(k) within 1e-5 in every
+    bin with more than one mode, displacement grids within 1e-5 (RMS and max
+    error over the FP64 RMS and max);
+  * 1536^3: the 3D FFTs of the drop-in surface on a plane wave (the exact
+    transform is one mode of height N^3/2), FP32 and FP64, which pins the
+    radix-3 line transforms at full size;
+  * 1536^3: one FP32 realization (the lattice spacing 2000/1536 rounds in
+    float, so the moves gather 27 taps at the actual positions): mode counts
+    equal the exact lattice enumeration, P(k) at k < 0.03 h/Mpc follows the
+    input spectrum within cosmic variance, zero-mean overdensity, positions
+    inside the box.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+L, SEED, BINS = 2000.0, 42, 128
+
+
+def _free_cache():
+    import torch
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def test_1024_fp32_vs_fp64(gpu):
+    import torch
+    N = 1024
+    _free_cache()
+    print("case 1024 fp64 fused", flush=True)
+    ctx = gpu.IC(gpu.ICConfig(N=N, boxlen=L, seed=SEED, bins=BINS, precision=gpu.MITOS_FP64))
+    p64 = ctx.run()
+    psi64 = ctx.displacements()
+    ctx.close()
+    _free_cache()
+    print("case 1024 fp32 fused", flush=True)
+    ctx = gpu.IC(gpu.ICConfig(N=N, boxlen=L, seed=SEED, bins=BINS, precision=gpu.MITOS_FP32))
+    p32 = ctx.run()
+    psi32 = ctx.displacements()
+    ctx.close()
+    assert np.array_equal(p32["obs"], p64["obs"])
+    ok = p64["obs"] > 1
+    err = np.abs(p32["P"][ok] - p64["P"][ok]) / np.abs(p64["P"][ok])
+    print(f"  P(k) fp32 vs fp64: max {err.max():.2e}, rms {np.sqrt((err ** 2).mean()):.2e} "
+          f"over {ok.sum()} bins", flush=True)
+    assert err.max() < 1e-5
+    for i in range(3):
+        a = psi64[i]
+        d = psi32[i].double() - a
+        rms = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(a))
+        mx = float(d.abs().max() / a.abs().max())
+        print(f"  psi[{i}] fp32 vs fp64: rms {rms:.2e}, max {mx:.2e}", flush=True)
+        assert rms < 1e-5
+        assert mx < 1e-5
+        del d
+    del psi32, psi64
+    _free_cache()
+
+
+def _plane_wave(N, a, b, c, dtype):
+    """cos(2 pi (a x + b y + c z)/N) on the device, (N, N, N), built plane by plane"""
+    import torch
+    out = torch.empty((N, N, N), dtype=dtype, device="cuda")
+    y = torch.arange(N, device="cuda", dtype=torch.int64).view(N, 1)
+    z = torch.arange(N, device="cuda", dtype=torch.int64).view(1, N)
+    base = (b * y + c * z) % N
+    for x in range(N):
+        ph = (base + a * x) % N
+        out[x] = torch.cos(ph.double() * (2 * math.pi / N)).to(dtype)
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_1536_fft_plane_wave(gpu, prec):
+    import torch
+    N = 1536
+    a, b, c = 1, 511, 700
+    _free_cache()
+    print("case 1536 plane wave", prec, flush=True)
+    rt = torch.float32 if prec == "fp32" else torch.float64
+    ct = torch.complex64 if prec == "fp32" else torch.complex128
+    f = _plane_wave(N, a, b, c, rt)
+    F = torch.empty((N, N, N // 2 + 1), dtype=ct, device="cuda")
+    torch.cuda.synchronize()
+    L_ = gpu.lib()
+    r2c = L_.mitos_fft_r2c_float if prec == "fp32" else L_.mitos_fft_r2c
+    c2r = L_.mitos_fft_c2r_float if prec == "fp32" else L_.mitos_fft_c2r
+    assert r2c(N, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(F.data_ptr())) == 0
+    peak = F[a, b, c].item()
+    h = N ** 3 / 2
+    tol = 1e-5 if prec == "fp32" else 1e-12
+    print(f"  peak {peak} (exact {h})", flush=True)
+    assert abs(peak - h) < tol * h
+    F[a, b, c] = 0
+    rest = float(F.abs().max())
+    print(f"  largest other mode / peak {rest / h:.2e}", flush=True)
+    assert rest < tol * h
+    # back: c2r of the single mode is N^3 f
+    F.zero_()
+    F[a, b, c] = h
+    g = torch.empty_like(f)
+    torch.cuda.synchronize()
+    assert c2r(N, ctypes.c_void_p(F.data_ptr()), ctypes.c_void_p(g.data_ptr())) == 0
+    del F
+    _free_cache()
+    worst = 0.0
+    for x0 in range(0, N, 256):
+        d = (g[x0:x0 + 256].double() / N ** 3 - f[x0:x0 + 256].double()).abs().max()
+        worst = max(worst, float(d))
+    print(f"  c2r max abs err {worst:.2e}", flush=True)
+    assert worst < (1e-5 if prec == "fp32" else 1e-12)
+    gpu.lib().mitos_release_workspace()
+    del f, g
+    _free_cache()
+
+
+def _theory_bins(N, boxlen, bins, A, n_s, gamma, kmax):
+    """sum over the bin's modes (multiplicity 1 on z = 0 and N/2, else 2) of
+    A k^n_s T_BBKS(k)^2 / mode count, for modes with k < kmax"""
+    from oracle import oracle as O
+    dk = 2 * math.pi / boxlen
+    nmax = int(kmax / dk) + 1
+    tab = gpu_bin_table = None  # noqa: F841
+    sums = np.zeros(bins)
+    cnt = np.zeros(bins)
+    for nx in range(-nmax, nmax + 1):
+        for ny in range(-nmax, nmax + 1):
+            for nz in range(0, nmax + 1):
+                n2 = nx * nx + ny * ny + nz * nz
+                if n2 == 0:
+                    continue
+                k = math.sqrt(n2) * dk
+                if k >= kmax:
+                    continue
+                bidx = O.pk_bin(N, boxlen, bins, nx % N, ny % N, nz)
+                m = 1 if nz in (0, N // 2) else 2
+                T = O.bbks(k, gamma)
+                sums[bidx] += m * A * k ** n_s * T * T
+                cnt[bidx] += m
+    return sums, cnt
+
+
+def test_1536_fp32_pipeline(gpu, oracle):
+    import torch
+    N = 1536
+    _free_cache()
+    print("case 1536 fp32 fused", flush=True)
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=SEED, bins=BINS, precision=gpu.MITOS_FP32)
+    ctx = gpu.IC(cfg)
+    out = ctx.run()
+    print(f"  stage ms {[round(v, 1) for v in out['stage_ms']]}", flush=True)
+    # mode counts: exact lattice enumeration of the reference's bin formula
+    tab = gpu.pk_bin_table(N, L, BINS)
+    n = np.arange(-(N // 2) + 1, N // 2 + 1)
+    c1 = np.bincount((n * n), minlength=3 * (N // 2) ** 2 + 1).astype(np.int64)
+    c2 = np.convolve(c1, c1)[: 3 * (N // 2) ** 2 + 1]
+    zs = np.arange(0, N // 2 + 1)
+    mz = np.where((zs == 0) | (zs == N // 2), 1, 2)
+    cz = np.zeros(3 * (N // 2) ** 2 + 1, np.int64)
+    np.add.at(cz, zs * zs, mz)
+    cnt = np.convolve(c2, cz)[: 3 * (N // 2) ** 2 + 1]
+    obs = np.zeros(BINS, np.int64)
+    sel = tab >= 0
+    np.add.at(obs, tab[sel], cnt[sel])
+    assert np.array_equal(out["obs"], obs)
+    # low-k P(k) against the input spectrum
+    sums, c = _theory_bins(N, L, BINS, out["amplitude"], cfg.n_s, cfg.gamma, 0.03)
+    use = (c > 0) & (c == out["obs"])
+    th = sums[use] / c[use]
+    r = out["P"][use] / th - 1.0
+    chi2 = float(np.sum(r ** 2 * c[use] / 2) / use.sum())
+    print(f"  {use.sum()} low-k bins: P/P_in - 1 in [{r.min():.3f}, {r.max():.3f}], "
+          f"chi2/dof {chi2:.2f}", flush=True)
+    assert use.sum() >= 5
+    assert chi2 < 3.0
+    dens = ctx.density()
+    m = float(dens.double().mean())
+    del dens
+    X, Y, Z = ctx.positions()
+    inside = all(bool(((t >= 0) & (t < L)).all()) for t in (X, Y, Z))
+    del X, Y, Z
+    ctx.close()
+    _free_cache()
+    print(f"  density mean {m:.2e}, positions inside box: {inside}", flush=True)
+    assert abs(m) < 1e-6
+    assert inside

@@ -7,8 +7,9 @@ Full-size checks that do not need the CPU oracle to run the whole grid:
   * the fused FP64 pipeline against the reference call order through the
     drop-in functions: P(k) within 1e-10, identical bin counts, positions;
   * invariants: the CIC overdensity has zero mean, ids are the closed form.
-The whole-grid comparison against the CPU oracle (minutes of host time) runs
-when MITOS_SLOW=1.
+  * the whole realization against the CPU oracle's FP64 run at this size,
+    stored as a fixture (tests/golden/config512.npz: P(k), norms and samples
+    of the displacement grids and the overdensity, sampled positions).
 """
 import ctypes
 import math
@@ -103,22 +104,64 @@ def test_512_fused_fp64_vs_reference_order(gpu):
     assert abs(mean) < 1e-10  # fixed-point weights: each particle's sum is 1 +- 4 2^-41
 
 
-@pytest.mark.skipif(os.environ.get("MITOS_SLOW") != "1",
-                    reason="whole-grid CPU oracle at 512^3 takes minutes (MITOS_SLOW=1)")
-def test_512_fused_fp64_vs_cpu_oracle(gpu, oracle):
-    print("case 512 vs CPU oracle", flush=True)
-    ref = oracle.ic_run(N, L, SEED, bins=BINS, want_psi=False, want_density=True,
-                        threads=min(16, os.cpu_count() or 1))
+def test_512_fused_fp64_vs_cpu_oracle(gpu):
+    """Config 2 against the CPU oracle's FP64 reference-order run of the same
+    realization, held in tests/golden/config512.npz (make_config512.py):
+    bit-identical bin counts, P(k) within 1e-10 per bin, the three
+    displacement grids and the overdensity within 1e-10 (norm-relative, and
+    4096 sampled values each), positions within one float ulp."""
+    import torch
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "config512.npz"))
+    print("case 512 fused fp64 vs CPU oracle fixture", flush=True)
     cfg = gpu.ICConfig(N=N, boxlen=L, seed=SEED, bins=BINS, precision=gpu.MITOS_FP64)
     ctx = gpu.IC(cfg)
     out = ctx.run()
-    assert np.array_equal(out["obs"], ref["obs"])
-    ok = ref["obs"] > 0
-    err = np.abs(out["P"][ok] - ref["P"][ok]) / np.abs(ref["P"][ok])
-    print("  P(k) max rel err vs oracle", err.max(), flush=True)
+    assert np.array_equal(out["obs"], fx["obs"])
+    ok = fx["obs"] > 0
+    err = np.abs(out["P"][ok] - fx["P"][ok]) / np.abs(fx["P"][ok])
+    kerr = np.abs(out["k"][ok] - fx["k"][ok]) / fx["k"][ok]
+    print("  P(k) max rel err vs oracle", err.max(), "k", kerr.max(), flush=True)
     assert err.max() < 1e-10
-    dens = ctx.density().cpu().numpy()
-    de = np.linalg.norm(dens - ref["density"]) / np.linalg.norm(ref["density"])
-    print("  density rel err", de, flush=True)
-    assert de < 1e-6
+    assert kerr.max() < 1e-13
+    psi = ctx.displacements()  # (3, N, N, N) float64 on the device
+    for i in range(3):
+        flat = psi[i].reshape(-1)
+        nrm = float(torch.linalg.vector_norm(flat))
+        idx = torch.as_tensor(fx["psi_idx"][i], device=flat.device)
+        val = flat[idx].cpu().numpy()
+        en = abs(nrm - fx["psi_norm"][i]) / fx["psi_norm"][i]
+        es = np.abs(val - fx["psi_val"][i]).max() / (fx["psi_norm"][i] / N ** 1.5)
+        print(f"  psi[{i}] norm rel err {en:.2e}, sampled max err / rms {es:.2e}", flush=True)
+        assert en < 1e-10
+        assert es < 1e-9
+    del psi, flat
+    dens = ctx.density().reshape(-1)
+    dn = float(torch.linalg.vector_norm(dens))
+    dm = float(dens.mean())
+    idx = torch.as_tensor(fx["dens_idx"], device=dens.device)
+    dval = dens[idx].cpu().numpy()
+    dn_err = abs(dn - float(fx["dens_norm"])) / float(fx["dens_norm"])
+    ds_err = np.abs(dval - fx["dens_val"]).max() / (float(fx["dens_norm"]) / N ** 1.5)
+    print(f"  density norm rel err {dn_err:.2e}, sampled max err / rms {ds_err:.2e}, "
+          f"mean {dm:.2e}", flush=True)
+    assert dn_err < 1e-10
+    assert ds_err < 1e-6
+    assert abs(dm) < 1e-10
+    del dens
+    X, Y, Z = ctx.positions()
+    pidx = torch.as_tensor(fx["part_idx"], device=X.device)
+    for name, t in (("X", X), ("Y", Y), ("Z", Z)):
+        got = t[pidx].cpu().numpy()
+        ref = fx[name]
+        ulp = np.spacing(np.maximum(np.abs(ref), np.float32(L / 2)).astype(np.float32))
+        d = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+        d = np.minimum(d, L - d)
+        same = float(np.mean(got == ref))
+        csum = float(t.double().sum())
+        cs_err = abs(csum - float(fx["pos_checksum"]["XYZ".index(name)])) / (N ** 3 * L)
+        print(f"  {name}: {same:.6f} of sampled positions identical, max diff "
+              f"{(d / ulp).max():.1f} ulp; checksum rel err {cs_err:.1e}", flush=True)
+        assert same > 0.999
+        assert (d <= ulp).all()
+        assert cs_err < 1e-12
     ctx.close()

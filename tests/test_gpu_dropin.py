@@ -19,7 +19,8 @@ def rel(a, b):
 
 def test_fft_r2c_c2r_dg(gpu, oracle):
     rng = np.random.default_rng(1)
-    for N in SIZES + [128]:
+    # powers of two and the 3*2^k sizes (radix-3 line transforms)
+    for N in SIZES + [128, 24, 48, 96, 192]:
         print("case fft_dg N", N, flush=True)
         L = 100.0 + N
         g = gpu.Grid(N, L)
@@ -46,7 +47,7 @@ def test_serial_fft_float(gpu, oracle):
     import torch
     rng = np.random.default_rng(2)
     lib = gpu.lib()
-    for N in [16, 64, 256]:
+    for N in [16, 64, 256, 24, 96, 384]:
         print("case fft_float N", N, flush=True)
         a = rng.standard_normal((N, N, N)).astype(np.float32)
         da = torch.from_numpy(a).cuda()
@@ -210,7 +211,8 @@ def test_solve2lpt(gpu, oracle):
     assert rel(w.box()[:, :, :N], ow.box[:, :, :N]) < 1e-11
     # error codes: density in configuration space, aliased grids
     d.momentum_space = 0
-    assert gpu.solve2LPT(p, d, w, 1.0, -3.0 / 7.0) == 2
+    # src/2lpt.c:69-72 returns 1 for a density that is not in momentum space
+    assert gpu.solve2LPT(p, d, w, 1.0, -3.0 / 7.0) == 1
     d.momentum_space = 1
     assert gpu.solve2LPT(p, d, p, 1.0, -3.0 / 7.0) == 5
 

@@ -49,3 +49,44 @@ def test_96_realization_ensemble(gpu):
     for i in range(96):
         assert np.array_equal(ens.pk(i)["P"], first[i], equal_nan=True)
     ens.close()
+
+
+def test_ensemble_members_vs_oracle(gpu, oracle):
+    """Realizations of the 96-member ensemble (FP32) against the oracle's FP64
+    reference-order run of the same config: identical bin counts, P(k) within
+    the FP32 tolerance in every bin with more than one mode."""
+    ic = gpu
+    cfgs = ic.ensemble_configs()
+    pick = (0, 31, 32)
+    ens = ic.Ensemble([cfgs[i] for i in pick])
+    ens.run()
+    for j, i in enumerate(pick):
+        c = cfgs[i]
+        print("case ensemble member vs oracle", i, c.N, c.boxlen, c.seed, c.n_s, flush=True)
+        ref = oracle.ic_run(c.N, c.boxlen, c.seed, n_s=c.n_s, sigma8=c.sigma8, gamma=c.gamma,
+                            bins=c.bins)
+        got = ens.pk(j)
+        assert np.array_equal(got["obs"], ref["obs"])
+        ok = ref["obs"] > 1
+        err = np.abs(got["P"][ok] - ref["P"][ok]) / np.abs(ref["P"][ok])
+        print("  P(k) max rel err", err.max(), flush=True)
+        assert err.max() < 1e-5
+    ens.close()
+
+
+def test_ensemble_enqueue_on_caller_stream(gpu):
+    """A graph launch on a non-default torch stream, then an immediate P(k)
+    fetch: the fetch waits for that launch."""
+    import torch
+    ic = gpu
+    cfgs = [ic.ICConfig(N=64, boxlen=256.0, seed=s, bins=16, precision=ic.MITOS_FP32)
+            for s in (31, 32)]
+    ref = [_unbatched(ic, c) for c in cfgs]
+    ens = ic.Ensemble(cfgs)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        ens.enqueue(s.cuda_stream)
+        for i in range(2):
+            _check_same(ens.pk(i), ref[i])
+    ens.close()

@@ -23,7 +23,8 @@ def _pos_err(X, Xr, L):
 
 
 @pytest.mark.parametrize("N,L,seed,bins,run2lpt", [
-    (32, 128.0, 42, 16, True), (64, 256.0, 42, 32, True), (32, 256.0, 3, 16, False)])
+    (32, 128.0, 42, 16, True), (64, 256.0, 42, 32, True), (32, 256.0, 3, 16, False),
+    (24, 96.0, 5, 12, True), (48, 150.0, 6, 16, True), (96, 300.0, 7, 24, True)])
 def test_fused_fp64_vs_oracle(gpu, oracle, N, L, seed, bins, run2lpt):
     print("case fused fp64", N, L, seed, bins, run2lpt, flush=True)
     ref = _oracle_run(oracle, N, L, seed, bins, run2lpt)
@@ -53,7 +54,9 @@ def test_fused_fp64_vs_oracle(gpu, oracle, N, L, seed, bins, run2lpt):
     ic.close()
 
 
-@pytest.mark.parametrize("N,L,seed,bins", [(32, 128.0, 42, 16), (64, 256.0, 42, 32)])
+@pytest.mark.parametrize("N,L,seed,bins", [
+    (32, 128.0, 42, 16), (64, 256.0, 42, 32), (48, 192.0, 8, 16), (96, 300.0, 9, 24),
+    (32, 100.0 / 3.0, 10, 16), (48, 100.0, 11, 16)])
 def test_fused_fp32_vs_oracle(gpu, oracle, N, L, seed, bins):
     print("case fused fp32", N, L, seed, bins, flush=True)
     ref = _oracle_run(oracle, N, L, seed, bins)
@@ -69,6 +72,12 @@ def test_fused_fp32_vs_oracle(gpu, oracle, N, L, seed, bins):
     e = np.linalg.norm(psi - rpsi) / np.linalg.norm(rpsi)
     print("  psi rel err (fp32)", e, flush=True)
     assert e < 1e-5
+    # positions: lattice-stencil moves where the lattice sits on grid nodes,
+    # 27-tap moves at the actual float positions otherwise (L = 100/3, 100/48)
+    X, Y, Z = (t.cpu().numpy() for t in ic.positions())
+    pe = max(_pos_err(X, ref["X"], L), _pos_err(Y, ref["Y"], L), _pos_err(Z, ref["Z"], L))
+    print("  max position diff / cell (fp32)", pe * N / L, flush=True)
+    assert pe < 1e-4 * L / N
     ic.close()
 
 
@@ -199,3 +208,27 @@ def test_every_fft_schedule_variant_128(gpu, oracle):
                 assert nrel < 1e-10 and err.max() < 1e-6
             else:
                 assert err.max() < 1e-4
+
+
+def test_enqueue_on_caller_stream_then_run(gpu):
+    """A realization enqueued on a caller stream, then another run on the
+    context's own stream with a different seed, then the first again on the
+    caller stream: each waits for the previous one on the device, so every
+    P(k) equals that of a separate run."""
+    import torch
+    N, L, bins = 64, 256.0, 16
+    cfg_a = gpu.ICConfig(N=N, boxlen=L, seed=21, bins=bins, precision=gpu.MITOS_FP32)
+    cfg_b = gpu.ICConfig(N=N, boxlen=L, seed=22, bins=bins, precision=gpu.MITOS_FP32)
+    ic = gpu.IC(cfg_a)
+    ref_a = ic.run(cfg_a)
+    ref_b = ic.run(cfg_b)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ic.enqueue(cfg_a, stream=s.cuda_stream)
+    got_b = ic.run(cfg_b)
+    ic.enqueue(cfg_a, stream=s.cuda_stream)
+    got_a = ic.fetch_pk()
+    ok = ref_a["obs"] > 0
+    assert np.array_equal(got_b["P"][ok], ref_b["P"][ok])
+    assert np.array_equal(got_a["P"][ok], ref_a["P"][ok])
+    ic.close()
