@@ -52,22 +52,47 @@ def measured_traffic(stage: str, N: int):
     return st["traffic_bytes"] / st["launches"]
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(min_seconds: float = 10.0) -> dict:
-    """The oracle's reference-order FP64 pipeline on one host core."""
+    """Configuration 1 on the host: the oracle's reference-order FP64 pipeline
+    at 64^3 in a 256 Mpc/h box (seed 42, 32 bins), one process (one rank),
+    OpenMP only where the reference has it (the kernel loops and the particle
+    loops; src/fft.c:139,279, src/mitos.c:844,1072) and serial FFTs (the
+    reference's FFTW plans are single-threaded), repeated for min_seconds."""
     from oracle import oracle as O
     O.build()
-    n, L = 128, 250.0  # 1.95 Mpc/h cells, as 1024^3 in 2000 Mpc/h
+    n, L = 64, 256.0
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.ic_run(n, L, 42, bins=32, threads=threads, fft_threads=1)  # plans, page-in
     reps, t0 = 0, time.perf_counter()
+    stages = [0.0] * 7
     while True:
-        O.ic_run(n, L, 42, bins=32, threads=1)
+        r = O.ic_run(n, L, 42, bins=32, threads=threads, fft_threads=1)
+        stages = [a + b for a, b in zip(stages, r["stage_seconds"])]
         reps += 1
         el = time.perf_counter() - t0
         if el >= min_seconds:
             break
-    return {"value": reps * n ** 3 / el, "unit": "particles/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 reference-order pipeline (GRF, 19 FFTs, 2LPT, TSC, CIC, "
-                      f"P(k) 32 bins) at {n}^3, L={L} Mpc/h, seed 42, {reps} runs in {el:.1f} s, "
-                      "1 thread"}
+    names = ("grf", "transfer", "2lpt", "displacement_grids", "particles_tsc", "cic", "pk")
+    return {"value": reps * n ** 3 / el, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"configuration 1: oracle FP64 reference-order pipeline (GRF, 19 FFTs, "
+                      f"2LPT, TSC, CIC, P(k) 32 bins) at {n}^3, L={L:g} Mpc/h, seed 42, "
+                      f"{reps} runs in {el:.1f} s; 1 process (1 MPI rank), {threads} OpenMP "
+                      f"threads in the reference's OpenMP loops, serial FFTs",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "ranks": 1,
+            "fft_backend": "the oracle's own FP64 mixed-radix FFT (FFTW is not installed), "
+                           "single-threaded as the reference's FFTW plans",
+            "stage_seconds_per_run": {k: v / reps for k, v in zip(names, stages)}}
 
 
 def bench_ensemble(ic, torch, reps: int = 5) -> dict:
@@ -251,16 +276,25 @@ def main():
         torch.cuda.empty_cache()
         ok = r64["obs"] > 1
         pk_err = np.abs(r32["P"][ok] - r64["P"][ok]) / np.abs(r64["P"][ok])
-        log("[bench] fp64 plain (reference call order)")
+        log("[bench] fp64 plain (reference call order): warm-up, then timed")
+        ic.run_reference_order(cfg64)  # library workspaces, tables, twiddles
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         rp = ic.run_reference_order(cfg64)
         elp = time.perf_counter() - t0
         okp = rp["obs"] > 1
         pk_plain = np.abs(rp["P"][okp] - r64["P"][okp]) / np.abs(r64["P"][okp])
+        stage_names = ("noise", "transfer", "2lpt", "displacement", "particles", "cic", "pk",
+                       "total")
         out["fp64_plain"] = {"particles_per_s": N ** 3 / elp, "ms": 1e3 * elp,
-                             "pipeline": "reference call order, drop-in calls, 19 FFTs"}
-        out["fp64_fused"] = {"particles_per_s": N ** 3 / el64, "ms": 1e3 * el64}
+                             "stage_ms": dict(zip(stage_names, rp["stage_ms"])),
+                             "pipeline": "reference call order, drop-in calls, 19 FFTs; grids "
+                                         "allocated and freed by the calls as in the reference; "
+                                         "second run timed"}
+        out["fp64_fused"] = {"particles_per_s": N ** 3 / el64, "ms": 1e3 * el64,
+                             "stage_ms": dict(zip(stage_names, r64["stage_ms"]))}
         out["speedup_fp32_vs_fp64_plain"] = out["value"] / out["fp64_plain"]["particles_per_s"]
+        out["speedup_fp32_vs_fp64_fused"] = out["value"] / out["fp64_fused"]["particles_per_s"]
         out["precision"] = {
             "pk_max_rel_err_fp32_vs_fp64": float(pk_err.max()),
             "pk_rms_rel_err_fp32_vs_fp64": float(np.sqrt((pk_err ** 2).mean())),

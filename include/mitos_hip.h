@@ -142,6 +142,15 @@ double sampleNorm(rng_state *state);
  * the reference consumes, so later draws continue the same stream */
 int generate_complex_grf(struct distributed_grid *dg, rng_state *state);
 int enforce_hermiticity(struct distributed_grid *dg);
+/* include/grf_ngeniclike.h:29-30 -- the N-GenIC white noise of
+ * src/grf_ngeniclike.c:28-203 into the device slab [X0, X0+NX) of an
+ * N x N x (N/2+1) half grid (block_size unused, as in the reference): a
+ * RANLUX (ranlxd1) stream seeded with `seed` fills the column seed table,
+ * each column reseeds and draws its phases and amplitudes; Nyquist modes,
+ * DC and the z = N/2 plane are zero.  Returns 4 when the seed table cannot
+ * be allocated (src/grf_ngeniclike.c:68-69). */
+int generate_ngeniclike_grf(fftw_complex *fbox, int N, int NX, int X0, long int block_size,
+                            double boxlen, long int seed);
 
 /* include/fft.h:90-116 */
 void fft_wavevector(int x, int y, int z, int N, double delta_k, double *kx, double *ky,
@@ -165,7 +174,7 @@ int fft_c2r_dg(struct distributed_grid *dg);
 int fft_apply_kernel_dg(struct distributed_grid *dg_write, const struct distributed_grid *dg_read,
                         void (*compute)(struct kernel *the_kernel), const void *params);
 
-/* Kernel callbacks (include/fft_kernels.h:28-31, 96-180, 198-219).  They
+/* Kernel callbacks (include/fft_kernels.h:28-180, 198-219).  They
  * evaluate the kernel on the host when called directly; passed to an
  * fft_apply_kernel* function, their address selects the device kernel.  Any
  * other callback -- a caller's own kernel, or the static inline copy of a
@@ -176,6 +185,13 @@ int fft_apply_kernel_dg(struct distributed_grid *dg_write, const struct distribu
  * include/fft_kernels.h of this library declares the kernels extern, so a
  * unit compiled against it reaches the device kernels. */
 void kernel_constant(struct kernel *the_kernel);          /* params: const double* */
+void kernel_lowpass(struct kernel *the_kernel);           /* params: double k_max  */
+void kernel_hipass(struct kernel *the_kernel);            /* params: double k_min  */
+void kernel_tophat(struct kernel *the_kernel);            /* params: double[2]     */
+void kernel_elliptic_tophat(struct kernel *the_kernel);   /* params: double[4]     */
+void kernel_translate(struct kernel *the_kernel);         /* params: double[3]     */
+void kernel_gaussian(struct kernel *the_kernel);          /* params: double R      */
+void kernel_gaussian_inv(struct kernel *the_kernel);      /* params: double R      */
 void kernel_inv_poisson(struct kernel *the_kernel);       /* params: unused        */
 void kernel_dx(struct kernel *the_kernel);
 void kernel_dy(struct kernel *the_kernel);
@@ -255,6 +271,16 @@ int solvePoisson_dg(struct distributed_grid *dg);
 /* include/2lpt.h:27-29 */
 int solve2LPT(struct distributed_grid *potential, struct distributed_grid *density,
               struct distributed_grid *workspace, double factor1, double factor2);
+
+/* include/monge_ampere.h:27-29: cycles of the Monge-Ampere iteration
+ * det(I + D^2 phi) = 1 + delta on top of the Zel'dovich potential; returns 1
+ * for a density not in momentum space (src/monge_ampere.c:82-85) */
+int solveMongeAmpere(struct distributed_grid *potential, struct distributed_grid *density,
+                     struct distributed_grid *workspace, int cycles);
+/* the root-mean-square residual of each cycle of this thread's last
+ * solveMongeAmpere (the value src/monge_ampere.c:183-186 prints): copies up
+ * to n values, returns the number of cycles */
+int mitos_monge_ampere_residuals(double *rms_eps, int n);
 
 /* include/calc_powerspec.h:26-36 -- box1/box2 device, bin arrays host.  The
  * reference returns void; a failure leaves NaN bins and mitos_last_error() set */

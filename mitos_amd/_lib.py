@@ -125,6 +125,8 @@ SIGNATURES = [
     ("generate_complex_grf", ctypes.c_int, [ctypes.POINTER(DistributedGrid),
                                             ctypes.POINTER(RngState)]),
     ("enforce_hermiticity", ctypes.c_int, [ctypes.POINTER(DistributedGrid)]),
+    ("generate_ngeniclike_grf", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_long, ctypes.c_double, ctypes.c_long]),
     ("fft_wavevector", None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_double, c_double_p, c_double_p, c_double_p, c_double_p]),
     ("fft_normalize_r2c", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
@@ -141,6 +143,13 @@ SIGNATURES = [
     ("fft_apply_kernel_dg", ctypes.c_int, [ctypes.POINTER(DistributedGrid),
                                            ctypes.POINTER(DistributedGrid), vp, vp]),
     ("kernel_constant", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_lowpass", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_hipass", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_tophat", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_elliptic_tophat", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_translate", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_gaussian", None, [ctypes.POINTER(Kernel)]),
+    ("kernel_gaussian_inv", None, [ctypes.POINTER(Kernel)]),
     ("kernel_inv_poisson", None, [ctypes.POINTER(Kernel)]),
     ("kernel_dx", None, [ctypes.POINTER(Kernel)]),
     ("kernel_dy", None, [ctypes.POINTER(Kernel)]),
@@ -172,6 +181,8 @@ SIGNATURES = [
     ("solvePoisson", ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_double]),
     ("solvePoisson_dg", ctypes.c_int, [ctypes.POINTER(DistributedGrid)]),
     ("solve2LPT", ctypes.c_int, [ctypes.POINTER(DistributedGrid)] * 3 + [ctypes.c_double] * 2),
+    ("solveMongeAmpere", ctypes.c_int, [DG_P] * 3 + [ctypes.c_int]),
+    ("mitos_monge_ampere_residuals", ctypes.c_int, [c_double_p, ctypes.c_int]),
     ("calc_cross_powerspec", None, [ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_int,
                                     c_double_p, c_double_p, c_int_p]),
     ("calc_cross_powerspec_float", None, [ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_int,

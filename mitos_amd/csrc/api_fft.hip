@@ -63,6 +63,26 @@ int kspec_from_callback(void (*compute)(struct kernel*), const void* params, int
         const struct mitos_spectrum_params* p = (const struct mitos_spectrum_params*)params;
         ks->id = KID_BBKS;
         ks->p[0] = p->Gamma;
+    } else if (compute == kernel_lowpass || compute == kernel_hipass ||
+               compute == kernel_gaussian || compute == kernel_gaussian_inv) {
+        if (!params) return MITOS_ERR_ARG;
+        ks->id = compute == kernel_lowpass   ? KID_LOWPASS
+                 : compute == kernel_hipass  ? KID_HIPASS
+                 : compute == kernel_gaussian ? KID_GAUSSIAN
+                                              : KID_GAUSSIAN_INV;
+        ks->p[0] = *(const double*)params;
+    } else if (compute == kernel_tophat) {
+        if (!params) return MITOS_ERR_ARG;
+        ks->id = KID_TOPHAT;
+        memcpy(ks->p, params, 2 * sizeof(double));
+    } else if (compute == kernel_elliptic_tophat) {
+        if (!params) return MITOS_ERR_ARG;
+        ks->id = KID_ELLIPTIC_TOPHAT;
+        memcpy(ks->p, params, 4 * sizeof(double));
+    } else if (compute == kernel_translate) {
+        if (!params) return MITOS_ERR_ARG;
+        ks->id = KID_TRANSLATE;
+        memcpy(ks->p, params, 3 * sizeof(double));
     } else {
         return MITOS_ERR_UNSUPPORTED;
     }
@@ -229,6 +249,51 @@ __global__ void k_delta2(double* out, const double* const* h, long n) {
              d_dxz * d_dxz;
 }
 
+// src/monge_ampere.c:124-151: residual (1 + rho) - det(I + Hessian) of every
+// entry (padding included), and per-block sums of resid^2 and rho^2 (tree in
+// a fixed order, so the diagnostic does not depend on scheduling)
+__global__ void __launch_bounds__(256) k_ma_residual(double* out, const double* const* h,
+                                                     const double* rho, long n, double* part) {
+#pragma clang fp contract(off)
+    __shared__ double se[256], sn[256];
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0, q = 0;
+    if (k < n) {
+        const double d_dxx = h[0][k], d_dxy = h[1][k], d_dxz = h[2][k];
+        const double d_dyy = h[3][k], d_dyz = h[4][k], d_dzz = h[5][k];
+        const double M0 = 1 + d_dxx, M1 = d_dxy, M2 = d_dxz;
+        const double M3 = d_dxy, M4 = 1 + d_dyy, M5 = d_dyz;
+        const double M6 = d_dxz, M7 = d_dyz, M8 = 1 + d_dzz;
+        // det3 of src/monge_ampere.c:38-42
+        const double det = M0 * (M4 * M8 - M5 * M7) - M1 * (M3 * M8 - M5 * M6) +
+                           M2 * (M3 * M7 - M4 * M6);
+        const double r = rho[k];
+        const double resid = (1 + r) - det;
+        out[k] = resid;
+        e = resid * resid;
+        q = r * r;
+    }
+    se[threadIdx.x] = e;
+    sn[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sn[threadIdx.x] += sn[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = se[0];
+        part[2 * blockIdx.x + 1] = sn[0];
+    }
+}
+
+__global__ void k_add_inplace(double* a, const double* b, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] += b[k];
+}
+
 // 2lpt.c:131-133
 __global__ void k_combine(double* pot, const double* work, long n, double f1, double f2) {
 #pragma clang fp contract(off)
@@ -297,6 +362,31 @@ extern "C" void kernel_bbks_transfer(struct kernel* the_kernel) {
     double* k = kern_parts(the_kernel);
     k[0] = v.re;
     k[1] = v.im;
+}
+
+// host evaluation of a parameterised callback through its device formula
+static void host_value(void (*self)(struct kernel*), struct kernel* the_kernel) {
+    KSpec ks;
+    kspec_from_callback(self, the_kernel->params, 0, 0, &ks);
+    cx<double> v = ksp_value(ks, the_kernel->kx, the_kernel->ky, the_kernel->kz, the_kernel->k);
+    double* k = kern_parts(the_kernel);
+    k[0] = v.re;
+    k[1] = v.im;
+}
+extern "C" void kernel_lowpass(struct kernel* the_kernel) { host_value(kernel_lowpass, the_kernel); }
+extern "C" void kernel_hipass(struct kernel* the_kernel) { host_value(kernel_hipass, the_kernel); }
+extern "C" void kernel_tophat(struct kernel* the_kernel) { host_value(kernel_tophat, the_kernel); }
+extern "C" void kernel_elliptic_tophat(struct kernel* the_kernel) {
+    host_value(kernel_elliptic_tophat, the_kernel);
+}
+extern "C" void kernel_translate(struct kernel* the_kernel) {
+    host_value(kernel_translate, the_kernel);
+}
+extern "C" void kernel_gaussian(struct kernel* the_kernel) {
+    host_value(kernel_gaussian, the_kernel);
+}
+extern "C" void kernel_gaussian_inv(struct kernel* the_kernel) {
+    host_value(kernel_gaussian_inv, the_kernel);
 }
 
 // ---- FFTs on distributed grids (src/fft.c:219-254) ---------------------------------
@@ -520,4 +610,106 @@ extern "C" int solve2LPT(struct distributed_grid* potential, struct distributed_
     MITOS_TRY(set_last_error(launch_status()));
     MITOS_TRY(set_last_error(api_sync()));
     return fft_r2c_dg(potential);
+}
+
+// ---- Monge-Ampere (src/monge_ampere.c:50-183) -----------------------------------------------
+namespace {
+thread_local std::vector<double> g_ma_eps;
+}
+
+extern "C" int solveMongeAmpere(struct distributed_grid* potential,
+                                struct distributed_grid* density,
+                                struct distributed_grid* workspace_grid, int cycles) {
+    if (!potential || !density || !workspace_grid || cycles < 0)
+        return set_last_error(MITOS_ERR_ARG);
+    const int N = density->N;
+    // src/monge_ampere.c:68-74 (asserts in the reference)
+    if (potential->NX != density->NX || density->NX != workspace_grid->NX ||
+        potential->X0 != density->X0 || density->X0 != workspace_grid->X0 ||
+        potential->N != density->N || density->N != workspace_grid->N)
+        return set_last_error(MITOS_ERR_DIMS);
+    if (potential->box == density->box || workspace_grid->box == density->box ||
+        potential->box == workspace_grid->box)
+        return set_last_error(MITOS_ERR_ARG);
+    if (density->momentum_space != 1) return set_last_error(1);  // :82-85
+    g_ma_eps.clear();
+
+    void (*const derivatives[3])(struct kernel*) = {kernel_dx, kernel_dy, kernel_dz};
+    const int index_a[] = {0, 0, 0, 1, 1, 2};
+    const int index_b[] = {0, 1, 2, 1, 2, 2};
+    MITOS_TRY(fft_apply_kernel_dg(potential, density, kernel_inv_poisson, NULL));
+    MITOS_TRY(fft_r2c_dg(density));  // :91 (as src/2lpt.c:78)
+    if (cycles == 0) return MITOS_OK;
+
+    const long padded = (long)N * N * (N + 2);
+    const unsigned nb = nblk(padded);
+    double* hess[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double** hdev = nullptr;
+    double* part = nullptr;
+    struct distributed_grid d;
+    memset(&d, 0, sizeof(d));
+    int rc = MITOS_OK;
+    for (int j = 0; j < 6 && rc == MITOS_OK; j++)
+        if (hipMalloc(&hess[j], sizeof(double) * padded) != hipSuccess) rc = MITOS_ERR_ALLOC;
+    if (rc == MITOS_OK && (hipMalloc(&hdev, sizeof(double*) * 6) != hipSuccess ||
+                           hipMalloc(&part, sizeof(double) * 2 * nb) != hipSuccess))
+        rc = MITOS_ERR_ALLOC;
+    if (rc == MITOS_OK) rc = alloc_local_grid(&d, N, density->boxlen, density->comm);
+    if (rc == MITOS_OK &&
+        copy_sync(hdev, hess, sizeof(double*) * 6, hipMemcpyHostToDevice) != hipSuccess)
+        rc = MITOS_ERR_DEVICE;
+    std::vector<double> hp(2 * (size_t)nb);
+    for (int it = 0; it < cycles && rc == MITOS_OK; it++) {
+        // the six Hessian components (:103-117)
+        for (int j = 0; j < 6 && rc == MITOS_OK; j++) {
+            rc = fft_apply_kernel_dg(&d, potential, derivatives[index_a[j]], NULL);
+            if (rc == MITOS_OK) rc = fft_apply_kernel_dg(&d, &d, derivatives[index_b[j]], NULL);
+            if (rc == MITOS_OK) rc = fft_c2r_dg(&d);
+            if (rc == MITOS_OK &&
+                hipMemcpyAsync(hess[j], d.box, sizeof(double) * padded, hipMemcpyDeviceToDevice,
+                               api_stream()) != hipSuccess)
+                rc = MITOS_ERR_DEVICE;
+        }
+        if (rc != MITOS_OK) break;
+        // residuals (:120-151)
+        hipLaunchKernelGGL(k_ma_residual, dim3(nb), dim3(256), 0, api_stream(),
+                           workspace_grid->box, (const double* const*)hdev, density->box, padded,
+                           part);
+        rc = launch_status();
+        if (rc == MITOS_OK &&
+            (hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost,
+                            api_stream()) != hipSuccess ||
+             hipStreamSynchronize(api_stream()) != hipSuccess))
+            rc = MITOS_ERR_DEVICE;
+        if (rc != MITOS_OK) break;
+        double eps = 0, norm = 0;
+        for (unsigned b = 0; b < nb; b++) {
+            eps += hp[2 * b];
+            norm += hp[2 * b + 1];
+        }
+        // (:159-172) Poisson solve of the residual, added to the potential
+        rc = solvePoisson_dg(workspace_grid);
+        if (rc == MITOS_OK) rc = fft_c2r_dg(potential);
+        if (rc == MITOS_OK) {
+            hipLaunchKernelGGL(k_add_inplace, dim3(nb), dim3(256), 0, api_stream(),
+                               potential->box, workspace_grid->box, padded);
+            rc = launch_status();
+        }
+        if (rc == MITOS_OK) rc = api_sync();
+        if (rc == MITOS_OK) rc = fft_r2c_dg(potential);
+        // (:183-186) root-mean-square residual relative to the source
+        g_ma_eps.push_back(sqrt((eps / norm) / ((double)N * N * N)));
+    }
+    free_local_grid(&d);
+    if (hdev) hipFree(hdev);
+    if (part) hipFree(part);
+    for (int j = 0; j < 6; j++)
+        if (hess[j]) hipFree(hess[j]);
+    return set_last_error(rc);
+}
+
+extern "C" int mitos_monge_ampere_residuals(double* rms_eps, int n) {
+    const int m = (int)g_ma_eps.size();
+    for (int i = 0; i < n && i < m && rms_eps; i++) rms_eps[i] = g_ma_eps[i];
+    return m;
 }

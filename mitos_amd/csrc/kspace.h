@@ -52,6 +52,30 @@ __host__ __device__ __forceinline__ cx<double> ksp_value(const KSpec& ks, double
         case KID_BBKS:
             if (k == 0) return mk<double>(0, 0);
             return mk<double>(bbks_T(k, ks.p[0]), 0);
+        case KID_LOWPASS: return mk<double>((k < ks.p[0]) ? 1.0 : 0.0, 0);          // :33-37
+        case KID_HIPASS: return mk<double>((k > ks.p[0]) ? 1.0 : 0.0, 0);           // :39-43
+        case KID_TOPHAT:                                                             // :45-51
+            return mk<double>((k >= ks.p[0] && k <= ks.p[1]) ? 1.0 : 0.0, 0);
+        case KID_ELLIPTIC_TOPHAT: {                                                  // :54-68
+            const double rx = ks.p[0], ry = ks.p[1], rz = ks.p[2], eta2 = ks.p[3];
+            const double kr = kx * rx + ky * ry + kz * rz;
+            const double kk = kx * kx + ky * ky + kz * kz;
+            const double rr = rx * rx + ry * ry + rz * rz;
+            const double theta = kk * rr + (eta2 - 1) * kr * kr;
+            return mk<double>((theta <= 4 * M_PI * M_PI) ? 1.0 : 0.0, 0);
+        }
+        case KID_TRANSLATE: {                                                        // :70-79
+            const double ph = ks.p[0] * kx + ks.p[1] * ky + ks.p[2] * kz;
+            return mk<double>(cos(ph), sin(ph));
+        }
+        case KID_GAUSSIAN: {                                                         // :81-86
+            const double kR = k * ks.p[0];
+            return mk<double>(exp(-kR * kR), 0);
+        }
+        case KID_GAUSSIAN_INV: {                                                     // :88-93
+            const double kR = k * ks.p[0];
+            return mk<double>(exp(kR * kR), 0);
+        }
     }
     return mk<double>(0, 0);
 }

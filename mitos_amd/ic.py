@@ -119,7 +119,58 @@ def kernel_params(name: str, **kw):
         return _lib.HermiteKernParams(kw["order"], kw["N"], kw["boxlen"])
     if name in ("kernel_sqrt_power_law", "kernel_bbks_transfer"):
         return _lib.SpectrumParams(kw.get("A", 0.0), kw.get("n_s", 0.0), kw.get("Gamma", 0.0))
+    if name in ("kernel_lowpass", "kernel_hipass", "kernel_gaussian", "kernel_gaussian_inv"):
+        return ctypes.c_double(kw["value"])
+    if name in ("kernel_tophat", "kernel_elliptic_tophat", "kernel_translate"):
+        v = list(kw["values"])
+        return (ctypes.c_double * len(v))(*v)
     return None
+
+
+class PerturbTable:
+    """Transfer functions T_f(k, log tau) as ``struct perturb_data`` with its
+    ``struct perturb_spline`` (initPerturbSpline); the numpy arrays are kept
+    alive with the structs.  delta has shape (n_functions, tau_size, k_size)."""
+
+    def __init__(self, k, log_tau, delta, titles, acc_size: int = 1000):
+        self.k = np.ascontiguousarray(k, np.float64)
+        self.log_tau = np.ascontiguousarray(log_tau, np.float64)
+        self.delta = np.ascontiguousarray(delta, np.float64)
+        nf, nt, nk = self.delta.shape
+        assert nt == self.log_tau.size and nk == self.k.size and len(titles) == nf
+        self._titles = [t.encode() for t in titles]
+        self.titles = (ctypes.c_char_p * nf)(*self._titles)
+        dp = lambda a: a.ctypes.data_as(_lib.c_double_p)
+        self.data = _lib.PerturbData(nk, nt, nf, dp(self.delta), dp(self.k), dp(self.log_tau),
+                                     None, None, None, None, self.titles, None)
+        self.spline = _lib.PerturbSpline()
+        check(lib().initPerturbSpline(ctypes.byref(self.spline), acc_size,
+                                      ctypes.byref(self.data)), "initPerturbSpline")
+
+    def params(self, index_src: int, log_tau: float) -> "_lib.SplineParams":
+        """struct spline_params for kernel_(inv_)transfer_function at log_tau"""
+        ti = ctypes.c_int()
+        ut = ctypes.c_double()
+        lib().perturbSplineFindTau(ctypes.byref(self.spline), log_tau, ctypes.byref(ti),
+                                   ctypes.byref(ut))
+        return _lib.SplineParams(ctypes.pointer(self.spline), index_src, ti.value, ut.value)
+
+    def close(self) -> None:
+        if self.spline.k_acc_table:
+            lib().cleanPerturbSpline(ctypes.byref(self.spline))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generatePerturbationGrid(cosmo, table: PerturbTable, grf: Grid, grid: Grid, title: str,
+                             fname: str | None = None, rescale: float = 1.0) -> int:
+    return lib().generatePerturbationGrid(ctypes.byref(cosmo), ctypes.byref(table.spline),
+                                          grf.ref(), grid.ref(), title.encode(),
+                                          fname.encode() if fname else None, rescale)
 
 
 def fft_apply_kernel_dg(w: Grid, r: Grid, name: str, params=None) -> int:

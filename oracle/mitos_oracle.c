@@ -279,12 +279,29 @@ static void orc_fft1d(double complex *x, double complex *work, const orc_plan *p
     if (a != x) memcpy(x, a, sizeof(double complex) * n);
 }
 
+/* threads of the FFT loops: 0 = the OpenMP default.  The reference's FFTW
+ * plans are single-threaded (no fftw_init_threads, Makefile:7), so the CPU
+ * baseline sets 1 here and keeps OpenMP only where the reference has it
+ * (src/fft.c:139,279, src/mitos.c:844,1072). */
+static int orc_fft_nthreads = 0;
+static int orc_fft_team(void) {
+#ifdef _OPENMP
+    return orc_fft_nthreads > 0 ? orc_fft_nthreads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+int orc_set_fft_threads(int n) {
+    orc_fft_nthreads = n > 0 ? n : 0;
+    return orc_fft_team();
+}
+
 /* transform every line of a complex 3D array along one axis (stride in elements) */
 static void orc_fft_axis(double complex *data, int N, int H, int axis, int sign) {
     const orc_plan *p = orc_get_plan(N);
     /* axis 0: x (stride N*H), axis 1: y (stride H); lines indexed by the other two */
     const long stride = (axis == 0) ? (long)N * H : (long)H;
-    #pragma omp parallel
+    #pragma omp parallel num_threads(orc_fft_team())
     {
         double complex *line = (double complex *)malloc(sizeof(double complex) * N);
         double complex *work = (double complex *)malloc(sizeof(double complex) * N);
@@ -304,7 +321,7 @@ static void orc_fft_axis(double complex *data, int N, int H, int axis, int sign)
 void orc_fft_r2c(int N, const double *in, int rs, double complex *out) {
     const int H = N / 2 + 1;
     const orc_plan *p = orc_get_plan(N);
-    #pragma omp parallel
+    #pragma omp parallel num_threads(orc_fft_team())
     {
         double complex *line = (double complex *)malloc(sizeof(double complex) * N);
         double complex *work = (double complex *)malloc(sizeof(double complex) * N);
@@ -329,7 +346,7 @@ void orc_fft_c2r(int N, const double complex *in, double *out, int rs) {
     orc_fft_axis(tmp, N, H, 0, +1);
     orc_fft_axis(tmp, N, H, 1, +1);
     const orc_plan *p = orc_get_plan(N);
-    #pragma omp parallel
+    #pragma omp parallel num_threads(orc_fft_team())
     {
         double complex *line = (double complex *)malloc(sizeof(double complex) * N);
         double complex *work = (double complex *)malloc(sizeof(double complex) * N);
@@ -428,6 +445,30 @@ double orc_kernel_value_re(int kid, const double *params, double kx, double ky, 
     case ORC_K_BBKS:
         if (k == 0) return 0.0;
         return orc_bbks_transfer(k, params[0]);
+    /* include/fft_kernels.h:33-93 */
+    case ORC_K_LOWPASS: return (k < params[0]) ? 1.0 : 0.0;
+    case ORC_K_HIPASS: return (k > params[0]) ? 1.0 : 0.0;
+    case ORC_K_TOPHAT: return (k >= params[0] && k <= params[1]) ? 1.0 : 0.0;
+    case ORC_K_ELLIPTIC_TOPHAT: {
+        const double kr = kx * params[0] + ky * params[1] + kz * params[2];
+        const double kk = kx * kx + ky * ky + kz * kz;
+        const double rr = params[0] * params[0] + params[1] * params[1] + params[2] * params[2];
+        const double theta = kk * rr + (params[3] - 1) * kr * kr;
+        return (theta <= 4 * M_PI * M_PI) ? 1.0 : 0.0;
+    }
+    case ORC_K_TRANSLATE: {
+        const double complex e = cexp(I * (params[0] * kx + params[1] * ky + params[2] * kz));
+        *im = cimag(e);
+        return creal(e);
+    }
+    case ORC_K_GAUSSIAN: {
+        const double kR = k * params[0];
+        return exp(-kR * kR);
+    }
+    case ORC_K_GAUSSIAN_INV: {
+        const double kR = k * params[0];
+        return exp(kR * kR);
+    }
     }
     return 0.0;
 }
@@ -462,6 +503,261 @@ int orc_apply_kernel_dg(orc_grid *w, const orc_grid *r, int kid, const double *p
 int orc_apply_kernel(double complex *w, const double complex *r, int N, double boxlen, int kid,
                      const double *params) {
     orc_apply_kernel_arr(w, r, N, boxlen, kid, params);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RANLUX ranlxd1 (GSL gsl_rng_ranlxd1, used by src/grf_ngeniclike.c) and   */
+/* the N-GenIC white noise                                                    */
+/* ------------------------------------------------------------------------ */
+/* Restated from the published algorithm (Luescher 1994, the 48-bit double
+ * variant; GSL's luxury level 1 = 202 steps per 12 outputs, its 31-bit
+ * shift-register seeding).  GSL is absent here and its version is not
+ * pinned (Makefile:9): parity unpinned. */
+static void orc_ranlxd_seed(orc_ranlxd *s, unsigned long seed) {
+    const double one_bit = 1.0 / 281474976710656.0;
+    if (seed == 0) seed = 1;
+    int i = (int)(seed & 0xFFFFFFFFUL);
+    int xbit[31];
+    for (int k = 0; k < 31; ++k) {
+        xbit[k] = i % 2;
+        i /= 2;
+    }
+    int ibit = 0, jbit = 18;
+    for (int k = 0; k < 12; ++k) {
+        double x = 0;
+        for (int l = 1; l <= 48; ++l) {
+            const double y = (double)((xbit[ibit] + 1) % 2);
+            x += x + y;
+            xbit[ibit] = (xbit[ibit] + xbit[jbit]) % 2;
+            ibit = (ibit + 1) % 31;
+            jbit = (jbit + 1) % 31;
+        }
+        s->x[k] = one_bit * x;
+    }
+    s->carry = 0;
+    s->ir = 11;
+    s->jr = 7;
+    s->ir_old = 0;
+    s->pr = 202;
+}
+
+static void orc_ranlxd_refill(orc_ranlxd *s) {
+    const double one_bit = 1.0 / 281474976710656.0;
+    for (int k = 0; k < s->pr; ++k) {
+        double y = s->x[s->jr] - s->x[s->ir];
+        y -= s->carry;
+        if (y < 0) {
+            s->carry = one_bit;
+            y += 1;
+        } else {
+            s->carry = 0;
+        }
+        s->x[s->ir] = y;
+        s->ir = (s->ir + 1) % 12;
+        s->jr = (s->jr + 1) % 12;
+    }
+    s->ir_old = s->ir;
+}
+
+static double orc_ranlxd_uniform(orc_ranlxd *s) {
+    s->ir = (s->ir + 1) % 12;
+    if (s->ir == s->ir_old) orc_ranlxd_refill(s);
+    return s->x[s->ir];
+}
+
+void orc_ranlxd_draws(unsigned long seed, long n, double *out) {
+    orc_ranlxd s;
+    orc_ranlxd_seed(&s, seed);
+    for (long i = 0; i < n; i++) out[i] = orc_ranlxd_uniform(&s);
+}
+
+/* src/grf_ngeniclike.c:28-203, loop by loop (one rank: NX = N, X0 = 0) */
+int orc_generate_ngeniclike_grf(double complex *Cdata, int N, double boxlen, long seed) {
+    const long Nmesh = N, Nsample = N;
+    const double PI = M_PI, Box = boxlen;
+    double fac = pow(Box, 1.5) / sqrt(2);
+    fac *= sqrt(2);
+    orc_ranlxd rg;
+    orc_ranlxd_seed(&rg, (unsigned long)seed);
+    unsigned int *seedtable = (unsigned int *)malloc(Nmesh * Nmesh * sizeof(unsigned int));
+    if (!seedtable) return 4;
+    for (long i = 0; i < Nmesh / 2; i++) {
+        for (long j = 0; j < i; j++)
+            seedtable[i * Nmesh + j] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i + 1; j++)
+            seedtable[j * Nmesh + i] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i; j++)
+            seedtable[(Nmesh - 1 - i) * Nmesh + j] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i + 1; j++)
+            seedtable[(Nmesh - 1 - j) * Nmesh + i] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i; j++)
+            seedtable[i * Nmesh + (Nmesh - 1 - j)] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i + 1; j++)
+            seedtable[j * Nmesh + (Nmesh - 1 - i)] = 0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i; j++)
+            seedtable[(Nmesh - 1 - i) * Nmesh + (Nmesh - 1 - j)] =
+                0x7fffffff * orc_ranlxd_uniform(&rg);
+        for (long j = 0; j < i + 1; j++)
+            seedtable[(Nmesh - 1 - j) * Nmesh + (Nmesh - 1 - i)] =
+                0x7fffffff * orc_ranlxd_uniform(&rg);
+    }
+    for (int axes = 0; axes < 3; axes++) {
+        for (long i = 0; i < Nmesh * Nmesh * (Nmesh / 2 + 1); i++) Cdata[i] = 0;
+        for (long i = 0; i < Nmesh; i++) {
+            for (long j = 0; j < Nmesh; j++) {
+                orc_ranlxd_seed(&rg, seedtable[i * Nmesh + j]);
+                for (long k = 0; k < Nmesh / 2; k++) {
+                    const double phase = orc_ranlxd_uniform(&rg) * 2 * PI;
+                    double ampl;
+                    do
+                        ampl = orc_ranlxd_uniform(&rg);
+                    while (ampl == 0);
+                    if (i == Nmesh / 2 || j == Nmesh / 2 || k == Nmesh / 2) continue;
+                    if (i == 0 && j == 0 && k == 0) continue;
+                    double kvec[3];
+                    kvec[0] = (i < Nmesh / 2) ? i * 2 * PI / Box : -(Nmesh - i) * 2 * PI / Box;
+                    kvec[1] = (j < Nmesh / 2) ? j * 2 * PI / Box : -(Nmesh - j) * 2 * PI / Box;
+                    kvec[2] = (k < Nmesh / 2) ? k * 2 * PI / Box : -(Nmesh - k) * 2 * PI / Box;
+                    if (fabs(kvec[0]) * Box / (2 * PI) > Nsample / 2) continue;
+                    if (fabs(kvec[1]) * Box / (2 * PI) > Nsample / 2) continue;
+                    if (fabs(kvec[2]) * Box / (2 * PI) > Nsample / 2) continue;
+                    double p_of_k = 1.0;
+                    p_of_k *= -log(ampl);
+                    const double delta = fac * sqrt(p_of_k) / 1.0;
+                    const double re = delta * sin(phase), im = delta * cos(phase);
+                    const long H = Nmesh / 2 + 1;
+                    if (k > 0) {
+                        Cdata[(i * Nmesh + j) * H + k] = re + I * im;
+                    } else if (i == 0) {
+                        if (j >= Nmesh / 2) continue;
+                        const long jj = Nmesh - j;
+                        Cdata[(i * Nmesh + j) * H + k] = re + I * im;
+                        Cdata[(i * Nmesh + jj) * H + k] = re - I * im;
+                    } else {
+                        if (i >= Nmesh / 2) continue;
+                        long ii = Nmesh - i, jj = Nmesh - j;
+                        if (ii == Nmesh) ii = 0;
+                        if (jj == Nmesh) jj = 0;
+                        Cdata[(i * Nmesh + j) * H + k] = re + I * im;
+                        Cdata[(ii * Nmesh + jj) * H + k] = re - I * im;
+                    }
+                }
+            }
+        }
+    }
+    free(seedtable);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tabulated transfer functions                                              */
+/* ------------------------------------------------------------------------ */
+/* src/perturb_spline.c:30-64: for v_i = k_min + (i/acc_size)(k_max - k_min),
+ * the largest j with k_j < v_i (a plain scan, as the reference) */
+void orc_spline_acc_table(const double *k, int k_size, int acc_size, double *acc) {
+    const double k_min = k[0], k_max = k[k_size - 1];
+    for (int i = 0; i < acc_size; i++) {
+        const double u = (double)i / acc_size;
+        const double v = k_min + u * (k_max - k_min);
+        double maxJ = 0;
+        for (int j = 0; j < k_size; j++)
+            if (k[j] < v) maxJ = j;
+        acc[i] = maxJ;
+    }
+}
+
+/* src/perturb_spline.c:73-97 */
+void orc_spline_find_tau(const double *log_tau, int tau_size, double lt, int *index,
+                         double *u) {
+    if (lt < log_tau[0]) {
+        *index = 0;
+        *u = 0.0;
+        return;
+    }
+    if (lt >= log_tau[tau_size - 1]) {
+        *index = tau_size - 2;
+        *u = 1.0;
+        return;
+    }
+    for (int i = 1; i < tau_size; i++)
+        if (log_tau[i] >= lt) {
+            *index = i - 1;
+            break;
+        }
+    *u = (lt - log_tau[*index]) / (log_tau[*index + 1] - log_tau[*index]);
+}
+
+/* src/perturb_spline.c:100-138 (the scan stops at the last interval, where
+ * the reference would read past the table only for k outside it) */
+void orc_spline_find_k(const double *k, int k_size, const double *acc, int acc_size, double kk,
+                       int *index, double *u) {
+    const double k_min = k[0], k_max = k[k_size - 1];
+    if (kk > k_max) {
+        *index = k_size - 2;
+        *u = 1.0;
+        return;
+    }
+    const double w = (kk - k_min) / (k_max - k_min);
+    const int J = (int)floor(w * acc_size);
+    int i = (int)acc[J < acc_size ? (J < 0 ? 0 : J) : acc_size - 1];
+    for (; i < k_size - 2; i++)
+        if (kk >= k[i] && kk <= k[i + 1]) break;
+    *index = i;
+    *u = (kk - k[i]) / (k[i + 1] - k[i]);
+}
+
+/* src/perturb_spline.c:141-160 */
+double orc_spline_interp(const double *delta, int k_size, int tau_size, int k_index,
+                         int tau_index, double u_k, double u_tau, int index_src) {
+    const double *arr = delta + (size_t)index_src * k_size * tau_size;
+    const double T11 = arr[(size_t)k_size * tau_index + k_index];
+    const double T21 = arr[(size_t)k_size * tau_index + k_index + 1];
+    const double T12 = arr[(size_t)k_size * (tau_index + 1) + k_index];
+    const double T22 = arr[(size_t)k_size * (tau_index + 1) + k_index + 1];
+    return (1 - u_tau) * ((1 - u_k) * T11 + u_k * T21) + u_tau * ((1 - u_k) * T12 + u_k * T22);
+}
+
+/* include/fft_kernels.h:131-180 inside the loop of src/fft.c:257-302 */
+void orc_apply_transfer(double complex *w, const double complex *r, int N, double boxlen,
+                        const double *k, int k_size, const double *delta, int tau_size,
+                        int index_src, int tau_index, double u_tau, int acc_size, int invert) {
+    double *acc = (double *)malloc(sizeof(double) * acc_size);
+    orc_spline_acc_table(k, k_size, acc_size, acc);
+    const double dk = 2 * M_PI / boxlen;
+    #pragma omp parallel for
+    for (int x = 0; x < N; x++)
+        for (int y = 0; y < N; y++)
+            for (int z = 0; z <= N / 2; z++) {
+                double kx, ky, kz, kk, kern = 0;
+                orc_wavevector(x, y, z, N, dk, &kx, &ky, &kz, &kk);
+                if (kk != 0) {
+                    int ki;
+                    double uk;
+                    orc_spline_find_k(k, k_size, acc, acc_size, kk, &ki, &uk);
+                    kern = orc_spline_interp(delta, k_size, tau_size, ki, tau_index, uk, u_tau,
+                                             index_src);
+                    if (invert) kern = 1.0 / kern;
+                }
+                const size_t id = half_idx(x, y, z, N);
+                const double a = creal(r[id]), b = cimag(r[id]);
+                w[id] = (a * kern - b * 0.0) + I * (a * 0.0 + b * kern);
+            }
+    free(acc);
+}
+
+/* src/shrink_grids.c:55-75 */
+int orc_shrink_grid(double *out, const double *in, int M, int N) {
+    const int K = N / M;
+    if (K * M != N) return 1;
+    memset(out, 0, sizeof(double) * M * M * M);
+    const double factor = 1.0 / (K * K * K);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            for (int k = 0; k < N; k++) {
+                const int x = i / K, y = j / K, z = k / K;
+                out[((size_t)x * M + y) * M + z] += factor * in[((size_t)i * N + j) * N + k];
+            }
     return 0;
 }
 
@@ -507,6 +803,57 @@ int orc_solve_2lpt(orc_grid *pot, orc_grid *dens, orc_grid *work, double f1, dou
     orc_fft_c2r_dg(pot);
     for (size_t k = 0; k < chunk; k++) pot->box[k] = pot->box[k] * f1 + f2 * work->box[k];
     orc_fft_r2c_dg(pot);
+    return 0;
+}
+
+/* src/monge_ampere.c:38-42 */
+static double orc_det3(const double *M) {
+    return M[0] * (M[4] * M[8] - M[5] * M[7])
+         - M[1] * (M[3] * M[8] - M[5] * M[6])
+         + M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+/* src/monge_ampere.c:50-183; rms_eps[c] receives the residual of cycle c */
+int orc_solve_monge_ampere(orc_grid *pot, orc_grid *dens, orc_grid *work, int cycles,
+                           double *rms_eps) {
+    const int N = dens->N;
+    const size_t chunk = (size_t)N * N * (N + 2);
+    const int da[] = {ORC_K_DX, ORC_K_DY, ORC_K_DZ};
+    const int ia[] = {0, 0, 0, 1, 1, 2};
+    const int ib[] = {0, 1, 2, 1, 2, 2};
+    if (dens->momentum_space != 1) return 1;
+    orc_apply_kernel_dg(pot, dens, ORC_K_INV_POISSON, NULL);
+    orc_fft_r2c_dg(dens); /* monge_ampere.c:91 (as 2lpt.c:78) */
+    for (int it = 0; it < cycles; it++) {
+        double *hess[6];
+        for (int j = 0; j < 6; j++) {
+            orc_grid d;
+            orc_grid_alloc(&d, N, dens->boxlen);
+            orc_apply_kernel_dg(&d, pot, da[ia[j]], NULL);
+            orc_apply_kernel_dg(&d, &d, da[ib[j]], NULL);
+            orc_fft_c2r_dg(&d);
+            free(d.fbox);
+            hess[j] = d.box;
+        }
+        double eps = 0, norm = 0;
+        for (size_t k = 0; k < chunk; k++) {
+            const double M[9] = {1 + hess[0][k], hess[1][k], hess[2][k],
+                                 hess[1][k], 1 + hess[3][k], hess[4][k],
+                                 hess[2][k], hess[4][k], 1 + hess[5][k]};
+            const double det = orc_det3(M);
+            const double rho = dens->box[k];
+            const double resid = (1 + rho) - det;
+            work->box[k] = resid;
+            eps += resid * resid;
+            norm += rho * rho;
+        }
+        for (int j = 0; j < 6; j++) free(hess[j]);
+        orc_solve_poisson_dg(work);
+        orc_fft_c2r_dg(pot);
+        for (size_t k = 0; k < chunk; k++) pot->box[k] += work->box[k];
+        orc_fft_r2c_dg(pot);
+        if (rms_eps) rms_eps[it] = sqrt((eps / norm) / ((long long)N * N * N));
+    }
     return 0;
 }
 
@@ -609,6 +956,43 @@ void orc_displace_tsc(int N, double boxlen, const double *psi_x, const double *p
             if (dir == 0) X[i] -= disp;
             else if (dir == 1) Y[i] -= disp;
             else Z[i] -= disp;
+        }
+    }
+    #pragma omp parallel for
+    for (long long i = 0; i < npart; i++) {
+        X[i] = orc_fwrap(X[i], boxlen);
+        Y[i] = orc_fwrap(Y[i], boxlen);
+        Z[i] = orc_fwrap(Z[i], boxlen);
+    }
+}
+
+/* src/mitos.c:827-863 (moves), :867-907 (velocities at the moved positions,
+ * before the wrap), :1072-1077 (wrap); psi or vel may be NULL */
+void orc_displace_velocity_tsc(int N, double boxlen, const double *const *psi,
+                               const double *const *vel, long long npart, float *X, float *Y,
+                               float *Z, float *VX, float *VY, float *VZ) {
+    if (psi) {
+        for (int dir = 0; dir < 3; dir++) {
+            #pragma omp parallel for
+            for (long long i = 0; i < npart; i++) {
+                double x = X[i], y = Y[i], z = Z[i];
+                double disp = orc_grid_tsc(psi[dir], N, boxlen, x, y, z);
+                if (dir == 0) X[i] -= disp;
+                else if (dir == 1) Y[i] -= disp;
+                else Z[i] -= disp;
+            }
+        }
+    }
+    if (vel) {
+        for (int dir = 0; dir < 3; dir++) {
+            #pragma omp parallel for
+            for (long long i = 0; i < npart; i++) {
+                double x = X[i], y = Y[i], z = Z[i];
+                double v = orc_grid_tsc(vel[dir], N, boxlen, x, y, z);
+                if (dir == 0) VX[i] = v;
+                else if (dir == 1) VY[i] = v;
+                else VZ[i] = v;
+            }
         }
     }
     #pragma omp parallel for

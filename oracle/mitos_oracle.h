@@ -75,7 +75,14 @@ enum {
     ORC_K_POWER_NO_TRANSFER = 5, /* params A_s, n_s, k_pivot      (:116-121, src/primordial.c:25-33) */
     ORC_K_UNDO_WINDOW = 6,     /* params order, N, boxlen         (:198-219)*/
     ORC_K_SQRT_POWER_LAW = 7,  /* params A, n_s: sqrt(A k^n_s), 0 at k=0 */
-    ORC_K_BBKS = 8             /* params Gamma: T_BBKS(k), 0 at k=0 */
+    ORC_K_BBKS = 8,            /* params Gamma: T_BBKS(k), 0 at k=0 */
+    ORC_K_LOWPASS = 9,         /* params k_max                    (:33-37)  */
+    ORC_K_HIPASS = 10,         /* params k_min                    (:39-43)  */
+    ORC_K_TOPHAT = 11,         /* params k_min, k_max             (:45-51)  */
+    ORC_K_ELLIPTIC_TOPHAT = 12, /* params rx, ry, rz, eta2        (:54-68)  */
+    ORC_K_TRANSLATE = 13,      /* params rx, ry, rz               (:70-79)  */
+    ORC_K_GAUSSIAN = 14,       /* params R                        (:81-86)  */
+    ORC_K_GAUSSIAN_INV = 15    /* params R                        (:88-93)  */
 };
 double orc_kernel_value_re(int kid, const double *params, double kx, double ky,
                            double kz, double k, double *im);
@@ -86,9 +93,40 @@ int orc_apply_kernel(double complex *w, const double complex *r, int N, double b
                      int kid, const double *params);
 int orc_normalize_r2c(double complex *arr, int N, double boxlen); /* src/fft.c:38-49 */
 
+/* ---- RANLUX ranlxd1 and the N-GenIC noise: src/grf_ngeniclike.c:28-203 ---- */
+typedef struct {
+    double x[12];
+    double carry;
+    int ir, jr, ir_old, pr;
+} orc_ranlxd;
+void orc_ranlxd_draws(unsigned long seed, long n, double *out);
+int orc_generate_ngeniclike_grf(double complex *fbox, int N, double boxlen, long seed);
+
+/* ---- tabulated transfer functions: src/perturb_spline.c:30-169 ---- */
+/* acceleration table of initPerturbSpline (:30-64) */
+void orc_spline_acc_table(const double *k, int k_size, int acc_size, double *acc);
+/* perturbSplineFindTau (:73-97), perturbSplineFindK (:100-138) */
+void orc_spline_find_tau(const double *log_tau, int tau_size, double lt, int *index,
+                         double *u);
+void orc_spline_find_k(const double *k, int k_size, const double *acc, int acc_size, double kk,
+                       int *index, double *u);
+/* perturbSplineInterp (:141-160) */
+double orc_spline_interp(const double *delta, int k_size, int tau_size, int k_index,
+                         int tau_index, double u_k, double u_tau, int index_src);
+/* kernel_(inv_)transfer_function (include/fft_kernels.h:131-180) applied to
+ * the half grid r (src/fft.c:257-302) */
+void orc_apply_transfer(double complex *w, const double complex *r, int N, double boxlen,
+                        const double *k, int k_size, const double *delta, int tau_size,
+                        int index_src, int tau_index, double u_tau, int acc_size, int invert);
+/* shrinkGrid (src/shrink_grids.c:55-75), unpadded N^3 -> M^3 */
+int orc_shrink_grid(double *out, const double *in, int M, int N);
+
 /* ---- Poisson / 2LPT: src/poisson.c:58-70, src/2lpt.c:38-143 ---- */
 int orc_solve_poisson_dg(orc_grid *g);
 int orc_solve_2lpt(orc_grid *pot, orc_grid *dens, orc_grid *work, double f1, double f2);
+/* src/monge_ampere.c:50-183; rms_eps (may be NULL) gets one value per cycle */
+int orc_solve_monge_ampere(orc_grid *pot, orc_grid *dens, orc_grid *work, int cycles,
+                           double *rms_eps);
 
 /* ---- spectrum: BBKS transfer function, sigma8 normalisation ---- */
 double orc_bbks_transfer(double k, double Gamma);
@@ -102,6 +140,9 @@ void orc_lattice(int M, double boxlen, long long first_id,
 double orc_grid_tsc(const double *box_padded, int N, double boxlen,
                     double x, double y, double z);
 /* src/mitos.c:827-863 then :1072-1077 */
+void orc_displace_velocity_tsc(int N, double boxlen, const double *const *psi,
+                               const double *const *vel, long long npart, float *X, float *Y,
+                               float *Z, float *VX, float *VY, float *VZ);
 void orc_displace_tsc(int N, double boxlen, const double *psi_x, const double *psi_y,
                       const double *psi_z, long long npart, float *X, float *Y, float *Z);
 /* analyse/src/mitos_read.c:278-334 (double CIC into unpadded N^3 grid);
@@ -141,6 +182,11 @@ typedef struct {
 } orc_ic_outputs;
 
 int orc_ic_run(const orc_ic_config *cfg, orc_ic_outputs *out);
+/* OpenMP threads of the loops (orc_set_threads) and of the FFTs alone
+ * (orc_set_fft_threads: 0 = the same as the loops, 1 = serial as FFTW's
+ * plans in the reference) */
+int orc_set_threads(int n);
+int orc_set_fft_threads(int n);
 
 #ifdef __cplusplus
 }

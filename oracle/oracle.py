@@ -106,12 +106,37 @@ def lib():
         L.orc_ic_run.argtypes = [ctypes.POINTER(OrcIcConfig), ctypes.POINTER(OrcIcOutputs)]
         L.orc_set_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_fft_threads.restype = ctypes.c_int
+        L.orc_set_fft_threads.argtypes = [ctypes.c_int]
         L.orc_pk_bin.restype = ctypes.c_int
         L.orc_pk_bin.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int]
         L.orc_pk_lut_mismatches.restype = ctypes.c_longlong
         L.orc_pk_lut_mismatches.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                             c_int_p]
+        L.orc_spline_acc_table.argtypes = [c_double_p, ctypes.c_int, ctypes.c_int, c_double_p]
+        L.orc_spline_find_tau.argtypes = [c_double_p, ctypes.c_int, ctypes.c_double, c_int_p,
+                                          c_double_p]
+        L.orc_spline_find_k.argtypes = [c_double_p, ctypes.c_int, c_double_p, ctypes.c_int,
+                                        ctypes.c_double, c_int_p, c_double_p]
+        L.orc_spline_interp.restype = ctypes.c_double
+        L.orc_spline_interp.argtypes = [c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int]
+        L.orc_apply_transfer.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_double, c_double_p, ctypes.c_int, c_double_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_displace_velocity_tsc.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_longlong, c_float_p,
+                                                c_float_p, c_float_p, c_float_p, c_float_p,
+                                                c_float_p]
+        L.orc_solve_monge_ampere.argtypes = [ctypes.POINTER(OrcGrid)] * 3 + [ctypes.c_int,
+                                                                            c_double_p]
+        L.orc_ranlxd_draws.argtypes = [ctypes.c_ulong, ctypes.c_long, c_double_p]
+        L.orc_generate_ngeniclike_grf.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_long]
+        L.orc_shrink_grid.argtypes = [c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -123,6 +148,71 @@ def _p(a, t):
 # kernel ids (mitos_oracle.h)
 K_CONSTANT, K_INV_POISSON, K_DX, K_DY, K_DZ = 0, 1, 2, 3, 4
 K_POWER_NO_TRANSFER, K_UNDO_WINDOW, K_SQRT_POWER_LAW, K_BBKS = 5, 6, 7, 8
+K_LOWPASS, K_HIPASS, K_TOPHAT, K_ELLIPTIC_TOPHAT = 9, 10, 11, 12
+K_TRANSLATE, K_GAUSSIAN, K_GAUSSIAN_INV = 13, 14, 15
+
+
+class Transfer:
+    """A table of transfer functions T_f(k, log tau) in the layout of
+    struct perturb_data (include/perturb_data.h:25-52): delta[f][tau][k]."""
+
+    def __init__(self, k, log_tau, delta, acc_size=1000):
+        self.k = np.ascontiguousarray(k, dtype=np.float64)
+        self.log_tau = np.ascontiguousarray(log_tau, dtype=np.float64)
+        self.delta = np.ascontiguousarray(delta, dtype=np.float64)
+        assert self.delta.shape[1:] == (self.log_tau.size, self.k.size)
+        self.acc_size = acc_size
+        self.acc = np.empty(acc_size)
+        lib().orc_spline_acc_table(_p(self.k, c_double_p), self.k.size, acc_size,
+                                   _p(self.acc, c_double_p))
+
+    def find_tau(self, lt):
+        i = ctypes.c_int()
+        u = ctypes.c_double()
+        lib().orc_spline_find_tau(_p(self.log_tau, c_double_p), self.log_tau.size, lt,
+                                  ctypes.byref(i), ctypes.byref(u))
+        return i.value, u.value
+
+    def find_k(self, kk):
+        i = ctypes.c_int()
+        u = ctypes.c_double()
+        lib().orc_spline_find_k(_p(self.k, c_double_p), self.k.size, _p(self.acc, c_double_p),
+                                self.acc_size, kk, ctypes.byref(i), ctypes.byref(u))
+        return i.value, u.value
+
+    def interp(self, ki, ti, uk, ut, f):
+        return lib().orc_spline_interp(_p(self.delta, c_double_p), self.k.size,
+                                       self.log_tau.size, ki, ti, uk, ut, f)
+
+    def apply(self, fbox, boxlen, f, tau_index, u_tau, invert=False):
+        N = fbox.shape[0]
+        r = np.ascontiguousarray(fbox, dtype=np.complex128)
+        w = np.empty_like(r)
+        lib().orc_apply_transfer(w.ctypes.data, r.ctypes.data, N, boxlen,
+                                 _p(self.k, c_double_p), self.k.size,
+                                 _p(self.delta, c_double_p), self.log_tau.size, f, tau_index,
+                                 u_tau, self.acc_size, int(invert))
+        return w
+
+
+def ranlxd_draws(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().orc_ranlxd_draws(seed, n, _p(out, c_double_p))
+    return out
+
+
+def ngeniclike_grf(N: int, boxlen: float, seed: int) -> np.ndarray:
+    out = np.zeros((N, N, N // 2 + 1), np.complex128)
+    assert lib().orc_generate_ngeniclike_grf(out.ctypes.data, N, boxlen, seed) == 0
+    return out
+
+
+def shrink_grid(box: np.ndarray, M: int) -> np.ndarray:
+    N = box.shape[0]
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    out = np.empty((M, M, M))
+    assert lib().orc_shrink_grid(_p(out, c_double_p), _p(b, c_double_p), M, N) == 0
+    return out
 
 
 def set_threads(n: int) -> int:
@@ -242,6 +332,28 @@ def displace_tsc(psi: np.ndarray, boxlen: float, X, Y, Z):
                            _p(Y, c_float_p), _p(Z, c_float_p))
 
 
+def displace_velocity_tsc(psi, vel, boxlen: float, X, Y, Z):
+    """psi, vel: (3, N, N, N+2) padded grids or None.  Positions updated in
+    place; returns the velocities (three float arrays) or None."""
+    N = (psi if psi is not None else vel).shape[1]
+    n = X.size
+    keep = []
+
+    def ptrs(g):
+        if g is None:
+            return None
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        keep.append(g)
+        arr = (ctypes.c_void_p * 3)(*[g[i].ctypes.data for i in range(3)])
+        keep.append(arr)
+        return ctypes.cast(arr, ctypes.c_void_p)
+    V = [np.zeros(n, np.float32) for _ in range(3)]
+    lib().orc_displace_velocity_tsc(N, boxlen, ptrs(psi), ptrs(vel), n, _p(X, c_float_p),
+                                    _p(Y, c_float_p), _p(Z, c_float_p), _p(V[0], c_float_p),
+                                    _p(V[1], c_float_p), _p(V[2], c_float_p))
+    return V if vel is not None else None
+
+
 def cic(N: int, boxlen: float, X, Y, Z, mass: float = 1.0):
     box = np.zeros((N, N, N), dtype=np.float64)
     tot = lib().orc_cic(N, boxlen, X.size, _p(X, c_float_p), _p(Y, c_float_p),
@@ -295,14 +407,25 @@ def solve_2lpt(pot: Grid, dens: Grid, work: Grid, f1: float, f2: float) -> int:
     return lib().orc_solve_2lpt(pot.ref(), dens.ref(), work.ref(), f1, f2)
 
 
+def solve_monge_ampere(pot: Grid, dens: Grid, work: Grid, cycles: int):
+    eps = np.zeros(max(cycles, 1))
+    rc = lib().orc_solve_monge_ampere(pot.ref(), dens.ref(), work.ref(), cycles,
+                                      _p(eps, c_double_p))
+    return rc, eps[:cycles]
+
+
 def ic_run(N: int, boxlen: float, seed: int, n_s: float = 0.965, sigma8: float = 0.81,
            gamma: float = 0.21, bins: int = 32, run2lpt: bool = True, f1: float = 1.0,
            f2: float = -3.0 / 7.0, want_psi: bool = False, want_density: bool = False,
-           want_delta_k: bool = False, threads: int | None = None):
-    """Reference-order FP64 pipeline; returns a dict of numpy outputs."""
+           want_delta_k: bool = False, threads: int | None = None,
+           fft_threads: int | None = None):
+    """Reference-order FP64 pipeline; returns a dict of numpy outputs.
+    threads: OpenMP threads of the loops; fft_threads: of the FFTs alone
+    (1 = serial, as the reference's FFTW plans; None = as the loops)."""
     L = lib()
     if threads is not None:
         L.orc_set_threads(threads)
+    L.orc_set_fft_threads(fft_threads or 0)
     n = N ** 3
     cfg = OrcIcConfig(N, boxlen, seed, n_s, sigma8, gamma, f1, f2, int(run2lpt), bins)
     res = {"X": np.empty(n, np.float32), "Y": np.empty(n, np.float32),

@@ -197,11 +197,11 @@ def test_config5_fp32_adjoint_256(gpu):
     _, ctx32, noise32, _, grad32 = _pk_setup(ic, N, L, bins, ic.MITOS_FP32)
     g32, l32 = grad32(noise32)
     print("  loss fp64", l64, "fp32", l32, flush=True)
-    assert abs(l32 / l64 - 1) < 1e-2
+    assert abs(l32 / l64 - 1) < 5e-5  # measured 5.3e-6
     a, b = g32.to(torch.complex128), g64
     rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
     print("  fp32 vs fp64 gradient rel err", rel, flush=True)
-    assert rel < 2e-2
+    assert rel < 1e-4  # measured 9.7e-6
     modes = _modes(N, 64, 3)
     fd = _fd(loss64, noise64, modes, 1e-3)
     gb = b.cpu().numpy()

@@ -189,3 +189,20 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+def test_c_caller_builds(tmp_path):
+    """the reference-style C caller compiles against include/mitos_hip.h and
+    the replacement include/fft_kernels.h and links against the library"""
+    import __graft_entry__ as g
+    out = g.build_c_caller(str(tmp_path / "dropin_caller"))
+    assert os.path.exists(out)
+
+
+def test_replacement_fft_kernels_header_is_exported(L):
+    """every callback the replacement include/fft_kernels.h declares is an
+    exported symbol of the library (so reference units reach the device)"""
+    src = open(os.path.join(ROOT, "include", "fft_kernels.h")).read()
+    names = re.findall(r"^void (kernel_\w+)\(", src, flags=re.M)
+    assert len(names) >= 16
+    assert all(hasattr(L, n) for n in names), [n for n in names if not hasattr(L, n)]

@@ -489,3 +489,309 @@ def test_adjoint_reference_forward_matches_oracle_pipeline(oracle):
     ok = ref["obs"] > 0
     assert np.array_equal(fw.obs[ok], ref["obs"][ok])
     assert np.max(np.abs(P[ok] / ref["P"][ok] - 1)) < 1e-4
+
+
+# ---- a11: kernel_power_no_transfer (include/fft_kernels.h:116-121) ---------------------
+def test_power_no_transfer_invariants(oracle):
+    """kern = sqrt(P) with P = k^4 P_phi the bare primordial spectrum of
+    src/primordial.c:25-33: P(k_pivot) = 2 pi^2 A_s k_pivot, the log-slope of
+    P is n_s (P = A_s k_p^(1-n_s) 2 pi^2 k^n_s), P(0) = 0 -- checked from
+    the oracle's values, without restating its expression"""
+    A_s, n_s, kp = 2.1e-9, 0.965, 0.05
+    N = 16
+    # boxes whose fundamental mode is k_pivot and 3 k_pivot
+    for L, scale in ((2 * math.pi / kp, 1.0), (2 * math.pi / (3 * kp), 3.0)):
+        fb = np.ones((N, N, N // 2 + 1), dtype=np.complex128)
+        out = oracle.apply_kernel(fb, L, oracle.K_POWER_NO_TRANSFER, (A_s, n_s, kp))
+        assert out[0, 0, 0] == 0
+        P1 = out[0, 0, 1].real ** 2  # k = k_fund
+        P2 = out[0, 0, 2].real ** 2  # k = 2 k_fund
+        assert np.all(out.imag == 0)
+        if scale == 1.0:
+            assert abs(P1 / (2 * math.pi ** 2 * A_s * kp) - 1) < 1e-14
+        assert abs(math.log(P2 / P1) / math.log(2.0) - n_s) < 1e-13
+        # isotropy: the same |k| along x, y and z gives the same value
+        assert out[1, 0, 0] == out[0, 1, 0] == out[0, 0, 1]
+    # between boxes: P(3 k_p) / P(k_p) = 3^n_s
+    Lb = 2 * math.pi / kp
+    fb = np.ones((N, N, N // 2 + 1), dtype=np.complex128)
+    out = oracle.apply_kernel(fb, Lb, oracle.K_POWER_NO_TRANSFER, (A_s, n_s, kp))
+    assert abs(out[0, 0, 3].real ** 2 / out[0, 0, 1].real ** 2 - 3 ** n_s) < 1e-13
+
+
+# ---- end to end: the realised spectrum recovers the input -----------------------------
+def _modes_per_bin(N, L, bins, Pfun):
+    """mean over each bin's modes (multiplicity 1 on z = 0, N/2, else 2) of
+    Pfun(k), with the bins of the reference's formula"""
+    b, k = _pk_bins_numpy(N, L, bins)
+    m = np.full(k.shape, 2.0)
+    m[:, :, 0] = 1
+    m[:, :, N // 2] = 1
+    Pk = np.where(k > 0, Pfun(np.where(k > 0, k, 1.0)), 0.0)
+    exp = np.zeros(bins)
+    cnt = np.zeros(bins)
+    for i in range(bins):
+        sel = b == i
+        exp[i] = (m[sel] * Pk[sel]).sum()
+        cnt[i] = m[sel].sum()
+    return np.where(cnt > 0, exp / np.maximum(cnt, 1), np.nan), cnt
+
+
+def test_grf_recovers_input_spectrum(oracle):
+    """tests/test_calc_powerspec.c:47-98: GRF, multiplied by sqrt(P(k)),
+    through c2r and r2c with the reference's normalisations, binned: every bin
+    with enough modes recovers P within cosmic variance (independent complex
+    modes: relative scatter sqrt(2/obs)); the chi^2 over the bins is ~1 per
+    bin.  This pins the GRF amplitude sqrt(L^3/2), the c2r division by L^3,
+    the r2c factor L^3/N^3 and the P(k) division by L^3 together."""
+    N, L, bins = 64, 400.0, 24
+    A, ns = 2.0e4, 0.97
+    f = oracle.grf(N, L, 17)
+    f = oracle.apply_kernel(f, L, oracle.K_SQRT_POWER_LAW, (A, ns))
+    box = oracle.fft_c2r(f) / L ** 3                     # src/fft.c:110-125
+    g = oracle.fft_r2c(box) * (L ** 3 / N ** 3)           # src/fft.c:38-49
+    kk, P, obs = oracle.powerspec(g, g, L, bins)
+    exp, cnt = _modes_per_bin(N, L, bins, lambda k: A * k ** ns)
+    ok = obs > 40
+    assert np.array_equal(obs, cnt.astype(obs.dtype))
+    r = P[ok] / exp[ok] - 1
+    assert np.all(np.abs(r) < 5 * np.sqrt(2.0 / obs[ok]))
+    chi2 = float(np.sum(r ** 2 * obs[ok] / 2) / ok.sum())
+    assert 0.3 < chi2 < 2.5, chi2
+
+
+def test_pipeline_low_k_recovers_input_spectrum(oracle):
+    """the oracle's whole FP64 pipeline (GRF, BBKS spectrum normalised to
+    sigma8, 2LPT, TSC-deconvolved displacements, lattice particles, CIC,
+    P(k) with the CIC window undone): at k well below the non-linear scale
+    the particle P(k) follows A k^n_s T_BBKS(k)^2 within cosmic variance"""
+    N, L, bins = 64, 2000.0, 24
+    res = oracle.ic_run(N, L, 42, bins=bins)
+    A = oracle.sigma8_amplitude(0.965, 0.21, 0.81)
+    exp, cnt = _modes_per_bin(N, L, bins,
+                              lambda k: A * k ** 0.965 * np.vectorize(oracle.bbks)(k, 0.21) ** 2)
+    ok = (res["obs"] > 20) & (res["k"] < 0.05)
+    assert ok.sum() >= 4
+    r = res["P"][ok] / exp[ok] - 1
+    assert np.all(np.abs(r) < 5 * np.sqrt(2.0 / res["obs"][ok]) + 0.02)
+
+
+# ---- a12: tabulated transfer functions (src/perturb_spline.c) -------------------------
+def _synthetic_table(rng, nk=57, nt=6, nf=3, kmin=0.004, kmax=3.0):
+    k = np.sort(np.exp(rng.uniform(np.log(kmin), np.log(kmax), nk)))
+    k[0], k[-1] = kmin, kmax
+    lt = np.sort(rng.uniform(-2.0, 3.0, nt))
+    delta = rng.standard_normal((nf, nt, nk)) + 3.0
+    return k, lt, delta
+
+
+def _chord(k, row, kk):
+    """brute force: the interval with k_i < kk <= k_{i+1} by searchsorted
+    (first interval at k_0), clamped to the last one above the table"""
+    if kk > k[-1]:
+        return row[-1]
+    i = max(int(np.searchsorted(k, kk, side="left")) - 1, 0)
+    u = (kk - k[i]) / (k[i + 1] - k[i])
+    return (1 - u) * row[i] + u * row[i + 1]
+
+
+def test_transfer_search_and_interp_brute_force(oracle):
+    rng = np.random.default_rng(21)
+    k, lt, delta = _synthetic_table(rng)
+    T = oracle.Transfer(k, lt, delta, acc_size=37)
+    # nodes: the table value itself, from either neighbouring interval
+    for j in (0, 1, 17, len(k) - 2, len(k) - 1):
+        ki, uk = T.find_k(k[j])
+        assert T.interp(ki, 2, uk, 0.0, 1) == delta[1, 2, j]
+    # random k and tau against the brute-force chord in k and the linear rule in tau
+    for kk in np.exp(rng.uniform(np.log(0.004), np.log(5.0), 400)):
+        ki, uk = T.find_k(kk)
+        for ltv in (lt[0] - 1, lt[1] + 0.3 * (lt[2] - lt[1]), lt[-1] + 2):
+            ti, ut = T.find_tau(ltv)
+            got = T.interp(ki, ti, uk, ut, 2)
+            lo = _chord(k, delta[2, ti], kk)
+            hi = _chord(k, delta[2, ti + 1], kk)
+            ref = (1 - ut) * lo + ut * hi
+            assert abs(got - ref) <= 1e-13 * abs(ref), (kk, ltv)
+    # tau outside the table: the first row / the last row
+    assert T.find_tau(lt[0] - 5) == (0, 0.0)
+    assert T.find_tau(lt[-1] + 5) == (len(lt) - 2, 1.0)
+    assert T.find_tau(lt[-1]) == (len(lt) - 2, 1.0)
+
+
+def test_transfer_kernel_on_grid(oracle):
+    """kernel_transfer_function applied to a half grid: DC -> 0, every other
+    mode multiplied by the interpolated T(|k|); the inverse kernel by 1/T"""
+    rng = np.random.default_rng(22)
+    k, lt, delta = _synthetic_table(rng, kmin=0.01, kmax=0.9)
+    T = oracle.Transfer(k, lt, delta)
+    N, L = 16, 120.0
+    fb = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    ti, ut = T.find_tau(0.4)
+    out = T.apply(fb, L, 1, ti, ut)
+    inv = T.apply(fb, L, 1, ti, ut, invert=True)
+    _, _, _, kk = _kgrid(N, L)
+    assert out[0, 0, 0] == 0 and inv[0, 0, 0] == 0
+    for idx in [(0, 0, 1), (3, 5, 2), (N // 2, N // 2, N // 2), (N - 1, 1, 7)]:
+        kv = kk[idx]
+        t = (1 - ut) * _chord(k, delta[1, ti], kv) + ut * _chord(k, delta[1, ti + 1], kv)
+        assert abs(out[idx] - fb[idx] * t) <= 1e-13 * abs(fb[idx] * t)
+        assert abs(inv[idx] - fb[idx] / t) <= 1e-13 * abs(fb[idx] / t)
+
+
+# ---- f4: shrinkGrid (src/shrink_grids.c:55-75) ------------------------------------------
+def test_shrink_grid_block_means(oracle):
+    rng = np.random.default_rng(23)
+    N, M = 12, 4
+    box = rng.standard_normal((N, N, N))
+    out = oracle.shrink_grid(box, M)
+    K = N // M
+    ref = box.reshape(M, K, M, K, M, K).mean(axis=(1, 3, 5))
+    assert np.abs(out - ref).max() < 1e-14
+    assert abs(out.sum() * K ** 3 - box.sum()) < 1e-11
+
+
+# ---- the remaining kernels of include/fft_kernels.h:33-93 -------------------------------
+def test_filter_kernels_invariants(oracle):
+    N, L = 16, 100.0
+    dk = 2 * math.pi / L
+    fb = np.ones((N, N, N // 2 + 1), dtype=np.complex128)
+    _, _, _, k = _kgrid(N, L)
+    n2 = np.rint((k / dk) ** 2).astype(np.int64)  # integer |n|^2 of every mode
+    lp = oracle.apply_kernel(fb, L, oracle.K_LOWPASS, (math.sqrt(20.5) * dk,))
+    hp = oracle.apply_kernel(fb, L, oracle.K_HIPASS, (math.sqrt(20.5) * dk,))
+    th = oracle.apply_kernel(fb, L, oracle.K_TOPHAT, (math.sqrt(5.5) * dk, math.sqrt(20.5) * dk))
+    assert np.array_equal(lp.real, (n2 < 20.5).astype(float))
+    assert np.array_equal(hp.real, (n2 > 20.5).astype(float))
+    assert np.array_equal(th.real, ((n2 > 5.5) & (n2 < 20.5)).astype(float))
+    # an elliptic top hat with eta2 = 1 is the sphere |k| |r| <= 2 pi
+    r = (3.0, 4.0, 12.0)  # |r| = 13
+    et = oracle.apply_kernel(fb, L, oracle.K_ELLIPTIC_TOPHAT, r + (1.0,))
+    assert np.array_equal(et.real, (k * 13.0 <= 2 * math.pi + 1e-12).astype(float))
+    # Gaussian and its inverse: product 1, value 1/e at k R = 1
+    R = 7.0
+    g = oracle.apply_kernel(fb, L, oracle.K_GAUSSIAN, (R,))
+    gi = oracle.apply_kernel(fb, L, oracle.K_GAUSSIAN_INV, (R,))
+    assert np.abs(g * gi - 1).max() < 1e-12
+    assert abs(g[0, 0, 0] - 1) == 0
+    # translation: unit modulus, composes additively, r = L is the identity
+    t1 = oracle.apply_kernel(fb, L, oracle.K_TRANSLATE, (1.5, -2.0, 0.25))
+    t2 = oracle.apply_kernel(t1, L, oracle.K_TRANSLATE, (0.5, 2.0, 0.75))
+    t12 = oracle.apply_kernel(fb, L, oracle.K_TRANSLATE, (2.0, 0.0, 1.0))
+    assert np.abs(np.abs(t1) - 1).max() < 1e-14
+    assert np.abs(t2 - t12).max() < 1e-12
+    tL = oracle.apply_kernel(fb, L, oracle.K_TRANSLATE, (L, 0.0, 0.0))
+    assert np.abs(tL - 1).max() < 1e-12
+
+
+# ---- f3: Monge-Ampere (src/monge_ampere.c:50-183) --------------------------------------
+def _ma_grids(oracle, N, L, delta):
+    dens, pot, work = oracle.Grid(N, L), oracle.Grid(N, L), oracle.Grid(N, L)
+    dens.box[:, :, :N] = delta
+    dens.fbox[:] = oracle.fft_r2c(delta) * (L ** 3 / N ** 3)
+    dens.s.momentum_space = b"\x01"
+    return dens, pot, work
+
+
+def test_monge_ampere_one_dimensional_is_poisson(oracle):
+    """for a density that varies along x only, det(I + D^2 phi) = 1 + phi_xx,
+    so the Zel'dovich (Poisson) potential already solves the equation: the
+    residual is at rounding level and further cycles leave phi unchanged"""
+    N, L = 16, 64.0
+    x = np.arange(N) * L / N
+    delta = np.broadcast_to((0.3 * np.cos(2 * np.pi * 2 * x / L) +
+                             0.2 * np.sin(2 * np.pi * 5 * x / L))[:, None, None],
+                            (N, N, N)).copy()
+    dens, pot, work = _ma_grids(oracle, N, L, delta)
+    rc, eps = oracle.solve_monge_ampere(pot, dens, work, 3)
+    assert rc == 0
+    assert np.all(eps < 1e-13)
+    dens0, pot0, work0 = _ma_grids(oracle, N, L, delta)
+    rc, _ = oracle.solve_monge_ampere(pot0, dens0, work0, 0)
+    assert np.abs(pot.fbox - pot0.fbox).max() < 1e-12 * np.abs(pot0.fbox).max()
+
+
+def test_monge_ampere_residual_decreases(oracle):
+    """a small three-dimensional perturbation: each cycle lowers the residual
+    of det(I + D^2 phi) = 1 + delta (by a factor of about 30 over six cycles)"""
+    N, L = 16, 64.0
+    rng = np.random.default_rng(12)
+    f = oracle.apply_kernel(rng.standard_normal((N, N, N // 2 + 1)) +
+                            1j * rng.standard_normal((N, N, N // 2 + 1)), L,
+                            oracle.K_GAUSSIAN, (4.0,))
+    delta = oracle.fft_c2r(f)
+    delta *= 0.2 / np.abs(delta).max()
+    dens, pot, work = _ma_grids(oracle, N, L, delta)
+    rc, eps = oracle.solve_monge_ampere(pot, dens, work, 6)
+    assert rc == 0
+    assert np.all(np.diff(eps) < 0), eps
+    assert eps[-1] < 0.1 * eps[0]
+    assert oracle.solve_monge_ampere(pot, work, dens, 1)[0] == 1  # not in momentum space
+
+
+# ---- a6: RANLUX ranlxd1 and generate_ngeniclike_grf (src/grf_ngeniclike.c) -------------
+def _ranlux48_integer_model(seed, n, p=202):
+    """RANLUX restated on 48-bit integers (not doubles): the seed register of
+    31 bits with taps 0 and 18 gives x_{-12..-1}; then x_t = x_{t-5} - x_{t-12}
+    - c mod 2^48 with borrow c; each refill of p steps delivers its last 12
+    values, oldest first"""
+    if seed == 0:
+        seed = 1
+    v = seed & 0xFFFFFFFF
+    bits = [(v >> k) & 1 for k in range(31)]
+    ib, jb = 0, 18
+    seq = []
+    for _ in range(12):
+        a = 0
+        for _ in range(48):
+            a = 2 * a + (bits[ib] ^ 1)
+            bits[ib] ^= bits[jb]
+            ib, jb = (ib + 1) % 31, (jb + 1) % 31
+        seq.append(a)
+    out, c = [], 0
+    while len(out) < n:
+        for _ in range(p):
+            d = seq[-5] - seq[-12] - c
+            c = 1 if d < 0 else 0
+            seq.append(d + (1 << 48) if d < 0 else d)
+        out.extend(seq[-12:])
+        seq = seq[-12:]
+    return np.array(out[:n], dtype=np.float64) / 2.0 ** 48
+
+
+def test_ranlxd_double_stream_matches_integer_model(oracle):
+    for seed in (1, 42, 12345, 0x7ffffffe):
+        got = oracle.ranlxd_draws(seed, 600)
+        assert np.array_equal(got, _ranlux48_integer_model(seed, 600)), seed
+    assert np.array_equal(oracle.ranlxd_draws(0, 30), oracle.ranlxd_draws(1, 30))
+    u = oracle.ranlxd_draws(7, 200000)
+    assert np.all((u >= 0) & (u < 1))
+    assert np.all(u * 2.0 ** 48 == np.floor(u * 2.0 ** 48))  # 48-bit fractions
+    assert abs(u.mean() - 0.5) < 5 / np.sqrt(12 * u.size)
+
+
+def test_ngeniclike_grf_structure(oracle):
+    """src/grf_ngeniclike.c:116-195: zero DC, Nyquist planes x, y = N/2 and
+    the z = N/2 plane; conjugate partners on the z = 0 plane; |delta|^2 /
+    L^3 = -ln(ampl) is Exp(1)-distributed, the phases uniform"""
+    N, L = 32, 150.0
+    f = oracle.ngeniclike_grf(N, L, 42)
+    assert f[0, 0, 0] == 0
+    assert np.all(f[N // 2] == 0) and np.all(f[:, N // 2] == 0) and np.all(f[:, :, N // 2] == 0)
+    z0 = f[:, :, 0]
+    m = (-np.arange(N)) % N
+    for x in range(N):
+        for y in range(N):
+            if x in (0, N // 2) or y == N // 2:
+                continue
+            assert z0[m[x], m[y]] == np.conj(z0[x, y])
+    for y in range(1, N // 2):
+        assert z0[0, N - y] == np.conj(z0[0, y])
+    inner = np.abs(f[1:N // 2, 1:N // 2, 1:N // 2]).ravel() ** 2 / L ** 3
+    assert abs(inner.mean() - 1) < 5 / np.sqrt(inner.size)
+    assert abs(np.mean(inner > 1) - np.exp(-1)) < 5 * 0.5 / np.sqrt(inner.size)
+    ph = np.angle(f[1:N // 2, 1:N // 2, 1:N // 2]).ravel()
+    assert abs(np.mean(np.cos(ph))) < 5 / np.sqrt(ph.size)
+    # the seed changes the field; the same seed repeats it
+    assert np.array_equal(f, oracle.ngeniclike_grf(N, L, 42))
+    assert not np.array_equal(f, oracle.ngeniclike_grf(N, L, 43))
