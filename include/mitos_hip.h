@@ -62,6 +62,7 @@ typedef int MPI_Comm;
 #define MITOS_ERR_ARG 5
 #define MITOS_ERR_ALLOC 6
 #define MITOS_ERR_REAL_SPACE 7
+#define MITOS_ERR_TOO_LARGE 8 /* input beyond a fixed scratch of an allocation-free call */
 
 /* include/distributed_grid.h:29-60 -- same fields, same order */
 struct distributed_grid {
@@ -403,6 +404,19 @@ int mitos_particles_export(long long npart, const float *X, const float *Y, cons
  * on thread order. */
 int mitos_cic_overdensity(int N, double boxlen, long long npart, const float *X,
                           const float *Y, const float *Z, double *box);
+
+/* No counterpart in the reference: mitos_cic_overdensity for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  It
+ * calls no allocation routine: the fixed-point grid is a fixed device array
+ * of MITOS_NOALLOC_MAX_CELLS cells (N^3 <= 2^21, N <= 128; 16 MiB held on
+ * every GPU the library runs on until the process ends), so larger grids
+ * return MITOS_ERR_TOO_LARGE.  Work is issued on `stream` (a hipStream_t)
+ * without a host wait; NULL issues it on the null stream and waits.  Calls
+ * that overlap on one GPU share the array and race.  Same result, bit for
+ * bit, as mitos_cic_overdensity. */
+#define MITOS_NOALLOC_MAX_CELLS (1L << 21)
+int mitos_cic_overdensity_noalloc(int N, double boxlen, long long npart, const float *X,
+                                  const float *Y, const float *Z, double *box, void *stream);
 
 /* ======================================================================== */
 /* Part 3: fused pipeline (GRF -> 2LPT -> displacement -> CIC -> P(k))      */

@@ -23,7 +23,7 @@ MITOS_FP32 = 1
 ERRORS = {
     0: "MITOS_OK", 1: "MITOS_ERR_DIMS", 2: "MITOS_ERR_NOT_MOMENTUM", 3: "MITOS_ERR_DEVICE",
     4: "MITOS_ERR_UNSUPPORTED", 5: "MITOS_ERR_ARG", 6: "MITOS_ERR_ALLOC",
-    7: "MITOS_ERR_REAL_SPACE",
+    7: "MITOS_ERR_REAL_SPACE", 8: "MITOS_ERR_TOO_LARGE",
 }
 
 
@@ -220,6 +220,8 @@ SIGNATURES = [
      [vp] * 6),
     ("mitos_particles_export", ctypes.c_int, [ctypes.c_longlong] + [vp] * 6 +
      [ctypes.c_double, ctypes.c_longlong] + [vp] * 4),
+    ("mitos_cic_overdensity_noalloc", ctypes.c_int, [ctypes.c_int, ctypes.c_double,
+                                                     ctypes.c_longlong, vp, vp, vp, vp, vp]),
     ("mitos_cic_overdensity", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_longlong,
                                              vp, vp, vp, vp]),
     ("mitos_sigma8_amplitude", ctypes.c_double, [ctypes.c_double] * 4),

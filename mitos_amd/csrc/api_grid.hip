@@ -235,6 +235,8 @@ extern "C" const char* mitos_strerror(int code) {
         case MITOS_ERR_ARG: return "invalid argument";
         case MITOS_ERR_ALLOC: return "device allocation failed";
         case MITOS_ERR_REAL_SPACE: return "input field is not in configuration space";
+        case MITOS_ERR_TOO_LARGE:
+            return "input larger than the fixed scratch of the allocation-free entry point";
     }
     return "unknown error";
 }

@@ -18,6 +18,7 @@
 #define MITOS_ERR_ARG 5           // invalid argument (null pointer, aliasing, range)
 #define MITOS_ERR_ALLOC 6         // device allocation failed
 #define MITOS_ERR_REAL_SPACE 7    // input grid not in configuration space
+#define MITOS_ERR_TOO_LARGE 8     // input beyond the fixed scratch of an allocation-free call
 #endif
 
 #define MITOS_HIP_CHECK(expr)                                                        \
