@@ -134,6 +134,38 @@ def bench_ensemble(ic, torch, reps: int = 5) -> dict:
                         "n_s 0.94..0.99, FP32, 2LPT, CIC, P(k) 32 bins"}
 
 
+def bench_adjoint(ic, torch, reps: int = 3) -> dict:
+    """Configuration 5: dL/d(white noise) of L = sum_b (P_b - P_target,b)^2 / b
+    at 256^3 in a 500 Mpc/h box (seed 7, target seed 8, 32 bins, FP32): one
+    forward (P(k)) and one backward per step, timed after a warm-up, and the
+    gradient's bit-reproducibility between two steps."""
+    from mitos_amd import adjoint
+    N, L, bins = 256, 500.0, 32
+    cfg = ic.ICConfig(N=N, boxlen=L, seed=7, bins=bins, precision=ic.MITOS_FP32)
+    ctx = ic.IC(cfg)
+    tgt = ctx.noise_forward(cfg, ic.white_noise(N, 8, ic.MITOS_FP32))["P"]
+    noise = ic.white_noise(N, 7, ic.MITOS_FP32)
+
+    def step():
+        leaf = noise.detach().requires_grad_(True)
+        adjoint.pk_loss(adjoint.noise_power_spectrum(leaf, ctx, cfg), tgt).backward()
+        return leaf.grad
+
+    g0 = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    same = bool(torch.equal(g0, g))
+    ctx.close()
+    log(f"[bench] adjoint 256^3 fp32: {1e3 * el:.1f} ms per forward + backward")
+    return {"workload": "configuration 5: 256^3, 500 Mpc/h, seed 7 (target seed 8), 32 bins, "
+                        "FP32, P(k) loss gradient w.r.t. the white noise",
+            "ms_per_gradient": 1e3 * el, "bit_reproducible": same}
+
+
 def pmc_pass(ic, torch, N, L, bins):
     """Dispatches profiled by the counter runs: realization 2 is the measured one."""
     cfg = ic.ICConfig(N=N, boxlen=L, seed=42, bins=bins, precision=ic.MITOS_FP32)
@@ -167,6 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 96-realization ensemble (one graph launch)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the configuration-5 gradient timing")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="one warm-up and one measured FP32 realization, then a plain copy "
                          "of 1 GiB (counter calibration); for rocprofv3 --pmc runs")
@@ -255,6 +289,8 @@ def main():
 
     if not a.no_ensemble:
         out["ensemble"] = bench_ensemble(ic, torch)
+    if not a.no_adjoint:
+        out["adjoint"] = bench_adjoint(ic, torch)
 
     if not a.no_fp64:
         # ---- FP64 fused and plain on the same realization ----------------------------------

@@ -72,11 +72,20 @@ __device__ __forceinline__ void wrap3(int c, int N, float invN, int o[3]) {
 // CIC adjoint at the final position, then the adjoint of the three moves.
 // sx, sy, sz: the stencilled displacement grids (halo rows of rs); grho, A*:
 // dense N^3 grids.
-template <typename T>
+// The y and z move adjoints scatter to neighbouring x (and y) lattice rows.
+// The scatter is made order-independent (bit-reproducible gradients) in
+// fixed point: pass 0 finds max|gy| and max|gz| (an integer maximum of the
+// doubles' bit patterns), pass 1 adds -g w in units of max * 2^-48 to 64-bit
+// integer grids (at most 2^15 maximal contributions per cell before overflow),
+// and k_adj_fixed_to_real converts them.
+constexpr int kAdjFixedBits = 48;
+
+template <typename T, int PASS>
 __global__ void __launch_bounds__(256)
     k_adj_particles(const T* __restrict__ sx, const T* __restrict__ sy,
                     const T* __restrict__ sz, unsigned rs, const T* __restrict__ grho, T* Ax,
-                    T* Ay, T* Az, int N, double boxlen, float spacing) {
+                    unsigned long long* AyI, unsigned long long* AzI,
+                    unsigned long long* gmax, int N, double boxlen, float spacing) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n = (long long)N * N * N;
     if (p >= n) return;
@@ -131,14 +140,37 @@ __global__ void __launch_bounds__(256)
     const T gz = gZ;
     const T gy = gY - gz * Fz_y * sN;
     const T gx = gX - gz * Fz_x * sN - gy * Fy_x * sN;
+    if constexpr (PASS == 0) {
+        // positive doubles order as their bit patterns
+        atomicMax(gmax, (unsigned long long)__double_as_longlong(fabs((double)gy)));
+        atomicMax(gmax + 1, (unsigned long long)__double_as_longlong(fabs((double)gz)));
+        return;
+    }
+    const double my = __longlong_as_double((long long)gmax[0]);
+    const double mz = __longlong_as_double((long long)gmax[1]);
+    const double sy_ = my > 0 ? ldexp(1.0, kAdjFixedBits) / my : 0.0;
+    const double sz_ = mz > 0 ? ldexp(1.0, kAdjFixedBits) / mz : 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const long ra = (long)ix[a] * N;
-        atomicAdd(Ay + (ra + qy) * N + qz, -gy * wx[a]);
+        atomicAdd(AyI + (ra + qy) * N + qz,
+                  (unsigned long long)llrint((double)(-gy * wx[a]) * sy_));
 #pragma unroll
-        for (int b = 0; b < 3; b++) atomicAdd(Az + (ra + iy[b]) * N + qz, -gz * wx[a] * wy[b]);
+        for (int b = 0; b < 3; b++)
+            atomicAdd(AzI + (ra + iy[b]) * N + qz,
+                      (unsigned long long)llrint((double)(-gz * wx[a] * wy[b]) * sz_));
     }
     Ax[p] = -gx;
+}
+
+template <typename T>
+__global__ void k_adj_fixed_to_real(const unsigned long long* __restrict__ I,
+                                    const unsigned long long* __restrict__ gmax, int which, T* A,
+                                    long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = __longlong_as_double((long long)gmax[which]);
+    A[i] = (T)((double)(long long)I[i] * ldexp(m, -kAdjFixedBits));
 }
 
 // k-space combine adjoint.  R_a = r2c of the displacement adjoints.  Forward:
@@ -334,11 +366,23 @@ static int backward(Pipe<T>* P, const mitos_ic_config& cfg, const cx<T>* noise,
     T* Ax = reinterpret_cast<T*>(P->b(1));
     T* Ay = reinterpret_cast<T*>(P->b(2));
     T* Az = reinterpret_cast<T*>(P->a(1));
-    hipLaunchKernelGGL((k_fill<T>), dim3(nblk(n3)), dim3(256), 0, s, Ay, n3, (T)0);
-    hipLaunchKernelGGL((k_fill<T>), dim3(nblk(n3)), dim3(256), 0, s, Az, n3, (T)0);
-    hipLaunchKernelGGL((k_adj_particles<T>), dim3(nblk(n3)), dim3(256), 0, s, P->psi(0),
-                       P->psi(1), P->psi(2), (unsigned)P->prs(), grho, Ax, Ay, Az, N,
-                       L, (float)L / N);
+    if (!P->adj_ws) MITOS_HIP_CHECK(hipMalloc(&P->adj_ws, sizeof(unsigned long long) *
+                                                          (2 * (size_t)n3 + 2)));
+    unsigned long long* AyI = P->adj_ws;
+    unsigned long long* AzI = AyI + n3;
+    unsigned long long* gmax = AzI + n3;
+    hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(nblk(2 * n3 + 2)), dim3(256), 0, s,
+                       AyI, 2 * n3 + 2, 0ULL);
+    hipLaunchKernelGGL((k_adj_particles<T, 0>), dim3(nblk(n3)), dim3(256), 0, s, P->psi(0),
+                       P->psi(1), P->psi(2), (unsigned)P->prs(), grho, Ax, AyI, AzI, gmax, N, L,
+                       (float)L / N);
+    hipLaunchKernelGGL((k_adj_particles<T, 1>), dim3(nblk(n3)), dim3(256), 0, s, P->psi(0),
+                       P->psi(1), P->psi(2), (unsigned)P->prs(), grho, Ax, AyI, AzI, gmax, N, L,
+                       (float)L / N);
+    hipLaunchKernelGGL((k_adj_fixed_to_real<T>), dim3(nblk(n3)), dim3(256), 0, s, AyI, gmax, 0,
+                       Ay, n3);
+    hipLaunchKernelGGL((k_adj_fixed_to_real<T>), dim3(nblk(n3)), dim3(256), 0, s, AzI, gmax, 1,
+                       Az, n3);
     MITOS_TRY(launch_status());
     // 3. back to k-space: R_a = r2c(A_a)
     cx<T>* R[3] = {P->b(3), P->b(4), P->b(5)};

@@ -573,7 +573,7 @@ extern "C" int solve2LPT(struct distributed_grid* potential, struct distributed_
     memset(&d, 0, sizeof(d));
     int rc = MITOS_OK;
     for (int j = 0; j < 6 && rc == MITOS_OK; j++)
-        if (hipMalloc(&hess[j], sizeof(double) * padded) != hipSuccess) rc = MITOS_ERR_ALLOC;
+        if (!(hess[j] = (double*)pool_alloc(sizeof(double) * padded))) rc = MITOS_ERR_ALLOC;
     if (rc == MITOS_OK) rc = alloc_local_grid(&d, N, density->boxlen, density->comm);
     double** hdev = nullptr;
     if (rc == MITOS_OK && hipMalloc(&hdev, sizeof(double*) * 6) != hipSuccess)
@@ -600,7 +600,7 @@ extern "C" int solve2LPT(struct distributed_grid* potential, struct distributed_
     free_local_grid(&d);
     if (hdev) hipFree(hdev);
     for (int j = 0; j < 6; j++)
-        if (hess[j]) hipFree(hess[j]);
+        pool_free(hess[j]);
     if (rc != MITOS_OK) return set_last_error(rc);
 
     MITOS_TRY(solvePoisson_dg(workspace_grid));
@@ -650,7 +650,7 @@ extern "C" int solveMongeAmpere(struct distributed_grid* potential,
     memset(&d, 0, sizeof(d));
     int rc = MITOS_OK;
     for (int j = 0; j < 6 && rc == MITOS_OK; j++)
-        if (hipMalloc(&hess[j], sizeof(double) * padded) != hipSuccess) rc = MITOS_ERR_ALLOC;
+        if (!(hess[j] = (double*)pool_alloc(sizeof(double) * padded))) rc = MITOS_ERR_ALLOC;
     if (rc == MITOS_OK && (hipMalloc(&hdev, sizeof(double*) * 6) != hipSuccess ||
                            hipMalloc(&part, sizeof(double) * 2 * nb) != hipSuccess))
         rc = MITOS_ERR_ALLOC;
@@ -704,7 +704,7 @@ extern "C" int solveMongeAmpere(struct distributed_grid* potential,
     if (hdev) hipFree(hdev);
     if (part) hipFree(part);
     for (int j = 0; j < 6; j++)
-        if (hess[j]) hipFree(hess[j]);
+        pool_free(hess[j]);
     return set_last_error(rc);
 }
 

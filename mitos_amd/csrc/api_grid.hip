@@ -102,12 +102,11 @@ extern "C" int alloc_local_grid(struct distributed_grid* dg, int N, double boxle
     dg->local_size = local;
     dg->fbox = nullptr;
     dg->box = nullptr;
-    void* f = nullptr;
-    void* b = nullptr;
-    if (hipMalloc(&f, sizeof(cx<double>) * local) != hipSuccess)
-        return set_last_error(MITOS_ERR_ALLOC);
-    if (hipMalloc(&b, sizeof(double) * 2 * local) != hipSuccess) {
-        hipFree(f);
+    void* f = pool_alloc(sizeof(cx<double>) * local);
+    if (!f) return set_last_error(MITOS_ERR_ALLOC);
+    void* b = pool_alloc(sizeof(double) * 2 * local);
+    if (!b) {
+        pool_free(f);
         return set_last_error(MITOS_ERR_ALLOC);
     }
     // zero-filled, so padding entries are defined; on the library stream and
@@ -116,8 +115,8 @@ extern "C" int alloc_local_grid(struct distributed_grid* dg, int N, double boxle
     if (hipMemsetAsync(f, 0, sizeof(cx<double>) * local, api_stream()) != hipSuccess ||
         hipMemsetAsync(b, 0, sizeof(double) * 2 * local, api_stream()) != hipSuccess ||
         api_sync() != MITOS_OK) {
-        hipFree(f);
-        hipFree(b);
+        pool_free(f);
+        pool_free(b);
         return set_last_error(MITOS_ERR_DEVICE);
     }
     dg->fbox = reinterpret_cast<fftw_complex*>(f);
@@ -127,20 +126,14 @@ extern "C" int alloc_local_grid(struct distributed_grid* dg, int N, double boxle
 
 extern "C" int free_local_real_grid(struct distributed_grid* dg) {
     if (!dg) return MITOS_ERR_ARG;
-    if (dg->box) {
-        hipDeviceSynchronize();
-        hipFree(dg->box);
-    }
+    if (dg->box) pool_free(dg->box);
     dg->box = nullptr;
     return MITOS_OK;
 }
 
 extern "C" int free_local_complex_grid(struct distributed_grid* dg) {
     if (!dg) return MITOS_ERR_ARG;
-    if (dg->fbox) {
-        hipDeviceSynchronize();
-        hipFree(dg->fbox);
-    }
+    if (dg->fbox) pool_free(dg->fbox);
     dg->fbox = nullptr;
     return MITOS_OK;
 }

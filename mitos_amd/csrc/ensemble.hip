@@ -158,6 +158,7 @@ static int ensemble_build(mitos_ensemble* e) {
 extern "C" int mitos_ensemble_create(const struct mitos_ic_config* cfgs, int n,
                                      mitos_ensemble** out) {
     if (!cfgs || n < 1 || !out) return set_last_error(MITOS_ERR_ARG);
+    pool_trim();
     mitos_ensemble* e = new mitos_ensemble();
     e->cfg.assign(cfgs, cfgs + n);
     const int rc = ensemble_build(e);

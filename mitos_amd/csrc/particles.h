@@ -378,6 +378,9 @@ __global__ void __launch_bounds__(NT)
     const int qz = (int)(bid - rowg * (unsigned)nch) * NT + threadIdx.x;
     if (qz >= N) return;
     const float invN = 1.0f / N;
+    // cells per length in double: the moved coordinates' cells within a
+    // double rounding of the reference's x*N/boxlen (float weights follow)
+    const double cpl = (double)N / boxlen;
     const unsigned plane = (unsigned)N * rs;
     const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT], qy[PPT];
@@ -411,7 +414,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         int cx;
-        tsc_axis_near_pos<T>((double)x[j] * N / boxlen, cx, wx[j]);
+        tsc_axis_near_pos<T>((double)x[j] * cpl, cx, wx[j]);
         tap_rows(wrapf(cx, N, invN), N, plane, ox[j]);
 #pragma unroll
         for (int a = 0; a < 3; a++) syv[j][a] = sy[ox[j][a] + yz[j]];
@@ -426,7 +429,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         int cy;
-        tsc_axis_near_pos<T>((double)y[j] * N / boxlen, cy, wy[j]);
+        tsc_axis_near_pos<T>((double)y[j] * cpl, cy, wy[j]);
         unsigned oy[3];
         tap_rows(wrapf(cy, N, invN), N, rs, oy);
 #pragma unroll
@@ -486,6 +489,7 @@ __global__ void __launch_bounds__(64)
     const int lane = threadIdx.x;
     const int qz = (int)(bid - rowg * (unsigned)nch) * 64 + lane;
     const float invN = 1.0f / N;
+    const double cpl = (double)N / boxlen;  // as in k_displace_lattice_s
     const unsigned plane = (unsigned)N * rs;
     const unsigned zo = MITOS_PSI_SO + qz;
     int qx[PPT];
@@ -514,7 +518,7 @@ __global__ void __launch_bounds__(64)
     int lo = 1 << 30, hi = -(1 << 30);
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        tsc_axis_near_pos<T>((double)x[j] * N / boxlen, cx[j], wx[j]);
+        tsc_axis_near_pos<T>((double)x[j] * cpl, cx[j], wx[j]);
         lo = cx[j] < lo ? cx[j] : lo;
         hi = cx[j] > hi ? cx[j] : hi;
     }
@@ -553,7 +557,7 @@ __global__ void __launch_bounds__(64)
     int ylo = 1 << 30, yhi = -(1 << 30);
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        tsc_axis_near_pos<T>((double)y[j] * N / boxlen, cy[j], wy[j]);
+        tsc_axis_near_pos<T>((double)y[j] * cpl, cy[j], wy[j]);
         ylo = cy[j] < ylo ? cy[j] : ylo;
         yhi = cy[j] > yhi ? cy[j] : yhi;
     }

@@ -344,6 +344,9 @@ struct Pipe {
     // workspace of the two-pass lattice deposit (float weights, N >= 128);
     // null: the deposit adds its windows with atomics
     void* cic_ws = nullptr;
+    // the adjoint's fixed-point scatter accumulators (two int64 grids and two
+    // maxima), allocated at the first backward run
+    unsigned long long* adj_ws = nullptr;
     // the FP32 displacement stages its taps in LDS (MITOS_TSC_DIRECT=1: reads
     // them from the grids, the same bits; for tests)
     bool tsc_staged = true;
@@ -516,6 +519,7 @@ struct Pipe {
         hipFree(pk_part);
         hipFree(pk_cnt);
         hipFree(cic_ws);
+        hipFree(adj_ws);
         for (int i = 0; i < 9; i++)
             if (ev[i]) hipEventDestroy(ev[i]);
         for (int i = 0; i < 33; i++)

@@ -36,6 +36,7 @@ extern "C" int mitos_ic_create(const struct mitos_ic_config* cfg, mitos_ic_conte
     if (!size_supported(cfg->N)) return set_last_error(MITOS_ERR_UNSUPPORTED);
     if (cfg->precision != MITOS_FP32 && cfg->precision != MITOS_FP64)
         return set_last_error(MITOS_ERR_ARG);
+    pool_trim();  // cached drop-in blocks give way to the pipeline's grids
     mitos_ic_context* c = new mitos_ic_context();
     c->N = cfg->N;
     c->precision = cfg->precision;

@@ -134,11 +134,14 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
     float *dX = X, *dY = Y, *dZ = Z;
     std::vector<void*> tmp;
     if (!dX || !dY || !dZ) {
-        if (hipMalloc(&dX, sizeof(float) * np) != hipSuccess ||
-            hipMalloc(&dY, sizeof(float) * np) != hipSuccess ||
-            hipMalloc(&dZ, sizeof(float) * np) != hipSuccess)
-            return MITOS_ERR_ALLOC;
+        dX = (float*)mitos::pool_alloc(sizeof(float) * np);
+        dY = (float*)mitos::pool_alloc(sizeof(float) * np);
+        dZ = (float*)mitos::pool_alloc(sizeof(float) * np);
         tmp = {dX, dY, dZ};
+        if (!dX || !dY || !dZ) {
+            for (void* p : tmp) mitos::pool_free(p);
+            return MITOS_ERR_ALLOC;
+        }
     }
     rc = mitos_lattice(N, L, 1, dX, dY, dZ, NULL);
     if (rc == MITOS_OK) rc = mitos_displace_tsc_dg(psi[0], psi[1], psi[2], np, dX, dY, dZ);
@@ -150,10 +153,11 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
     // analyse/src/mitos_read.c:278-419
     double* box = nullptr;
     fftw_complex* fbox = nullptr;
-    if (rc == MITOS_OK && (hipMalloc(&box, sizeof(double) * np) != hipSuccess ||
-                           hipMalloc(&fbox, sizeof(double) * 2 * N * (long)N * (N / 2 + 1)) !=
-                               hipSuccess))
-        rc = MITOS_ERR_ALLOC;
+    if (rc == MITOS_OK) {
+        box = (double*)mitos::pool_alloc(sizeof(double) * np);
+        fbox = (fftw_complex*)mitos::pool_alloc(sizeof(double) * 2 * N * (long)N * (N / 2 + 1));
+        if (!box || !fbox) rc = MITOS_ERR_ALLOC;
+    }
     if (rc == MITOS_OK) rc = mitos_cic_overdensity(N, L, np, dX, dY, dZ, box);
     t1 = now();
     stage[5] = (float)(1e3 * (t1 - t0));
@@ -169,9 +173,9 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
     }
     t1 = now();
     stage[6] = (float)(1e3 * (t1 - t0));
-    for (void* p : tmp) hipFree(p);
-    hipFree(box);
-    hipFree(fbox);
+    for (void* p : tmp) mitos::pool_free(p);
+    mitos::pool_free(box);
+    mitos::pool_free(fbox);
     if (rc != MITOS_OK) return rc;
     if (res) {
         for (int i = 0; i < cfg->bins; i++) {

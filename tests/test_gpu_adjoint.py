@@ -185,6 +185,21 @@ def test_density_adjoint_vs_finite_differences_fp64(gpu):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_adjoint_is_bit_reproducible(gpu, prec):
+    """the particle adjoint scatters in fixed point: two backward runs give
+    the same gradient bits"""
+    ic = gpu
+    p = ic.MITOS_FP32 if prec == "fp32" else ic.MITOS_FP64
+    print("case adjoint reproducible", prec, flush=True)
+    _, ctx, noise, _, grad_of = _pk_setup(ic, 64, 200.0, 16, p)
+    g1, l1 = grad_of(noise)
+    g2, l2 = grad_of(noise)
+    assert l1 == l2
+    assert torch.equal(g1, g2)
+    ctx.close()
+
+
 def test_config5_fp32_adjoint_256(gpu):
     # 256^3, box 500 Mpc/h, seed 7, target P(k) of seed 8, 32 bins, w_b = 1/b;
     # the FP32 gradient against the FP64 one (whole field) and against FP64
