@@ -622,6 +622,198 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// work-group minimum and maximum of an int (256 threads; `red` holds the
+// four waves' pairs)
+__device__ __forceinline__ void block_minmax256(int& lo, int& hi, int (*red)[2]) {
+    wave_minmax(lo, hi);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = lo;
+        red[wv][1] = hi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        lo = red[w][0] < lo ? red[w][0] : lo;
+        hi = red[w][1] > hi ? red[w][1] : hi;
+    }
+}
+
+// FP32 lattice displacement over 8 x 8 x 16 lattice tiles (x, y, z), one
+// work-group of 256 threads per tile; thread t moves the four x-consecutive
+// particles (x0 + 4 (t >> 7) + j, y0 + ((t >> 4) & 7), z0 + (t & 15)).
+// The taps of a tile's y and z moves lie in small windows of the displacement
+// grids (the displacement varies little over a compact tile): after the x move
+// the tile's x taps span [xlo - 1, xhi + 1].  Those rows of S_yz psi_y (the
+// tile's 8 y and 16 z) are loaded by the whole work-group into LDS, at most
+// U loads in flight per thread, and the y move reads its taps there; after
+// the y move the rows (x window) x (y window) of S_z psi_z follow the same way.
+// A window over MAXX x rows or MAXR rows is read from the grids directly.  The
+// weights, taps and order of the sums are those of k_displace_lattice_s, so
+// the positions are the same bits.  N must be a multiple of 16.
+template <typename T, int MAXX, int MAXR, int U>
+__global__ void __launch_bounds__(256)
+    k_displace_tile(const T* __restrict__ sx, const T* __restrict__ sy,
+                    const T* __restrict__ sz, unsigned rs, int N, T Lf, float spacing,
+                    double boxlen, float* __restrict__ X, float* __restrict__ Y,
+                    float* __restrict__ Z, int staged) {
+    static_assert(MAXX * 8 <= MAXR, "the y window fits the buffer");
+    constexpr int PP = 4;
+    __shared__ T buf[MAXR * 16];
+    __shared__ int redx[4][2], redy[4][2];
+    const int t = threadIdx.x;
+    const int zl = t & 15, yl = (t >> 4) & 7, xh = t >> 7;
+    const unsigned nzt = (unsigned)N / 16u, nxt = (unsigned)N / 8u;
+    // z tiles fastest, then x (the x halo of the next tile is still in L2)
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned zt = bid % nzt, r1 = bid / nzt;
+    const unsigned xt = r1 % nxt, yt = r1 / nxt;
+    const int qz = (int)zt * 16 + zl, qy = (int)yt * 8 + yl;
+    const int x0 = (int)xt * 8 + xh * PP;
+    const float invN = 1.0f / N;
+    const double cpl = (double)N / boxlen;  // as in k_displace_lattice_s
+    const unsigned plane = (unsigned)N * rs;
+    float x[PP], y[PP];
+    const float z = qz * spacing;  // src/particle.c:107-109
+    T sxv[PP];
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        x[j] = (x0 + j) * spacing;
+        y[j] = qy * spacing;
+        sxv[j] = sx[(unsigned)(x0 + j) * plane + (unsigned)qy * rs + qz];
+    }
+    // x move: the stencilled x displacement at the site
+#pragma unroll
+    for (int j = 0; j < PP; j++) x[j] = (float)((T)x[j] - sxv[j]);
+    T wx[PP][3];
+    int cx[PP];
+    int lo = 1 << 30, hi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        tsc_axis_near_pos<T>((double)x[j] * cpl, cx[j], wx[j]);
+        lo = cx[j] < lo ? cx[j] : lo;
+        hi = cx[j] > hi ? cx[j] : hi;
+    }
+    block_minmax256(lo, hi, redx);
+    const int nx = hi - lo + 3;  // x rows lo - 1 .. hi + 1
+    const bool stage_x = staged && nx <= MAXX;
+    // y move: TSC along x at (x', q_y, q_z)
+    T syv[PP][3];
+    if (stage_x) {
+        // window row xi (x = lo - 1 + xi) of this thread's (y, z) column at
+        // buf[(xi * 8 + yl) * 16 + zl]; the thread loads rows xh, xh + 2, ...
+        const unsigned col = (unsigned)qy * rs + qz;
+        for (int k0 = xh; k0 < nx; k0 += 2 * U) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int xi = k0 + 2 * u;
+                if (xi < nx) v[u] = sy[(unsigned)wrapf(lo - 1 + xi, N, invN) * plane + col];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int xi = k0 + 2 * u;
+                if (xi < nx) buf[(xi * 8 + yl) * 16 + zl] = v[u];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PP; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) syv[j][a] = buf[((cx[j] - lo + a) * 8 + yl) * 16 + zl];
+    } else {
+#pragma unroll
+        for (int j = 0; j < PP; j++) {
+            unsigned o[3];
+            tap_rows(wrapf(cx[j], N, invN), N, plane, o);
+#pragma unroll
+            for (int a = 0; a < 3; a++) syv[j][a] = sy[o[a] + (unsigned)qy * rs + qz];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PP; j++)
+        y[j] = (float)((T)y[j] -
+                       fma(wx[j][2], syv[j][2], fma(wx[j][1], syv[j][1], wx[j][0] * syv[j][0])));
+    // z move: TSC along x and y at (x', y', q_z)
+    T wy[PP][3];
+    int cy[PP];
+    int ylo = 1 << 30, yhi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        tsc_axis_near_pos<T>((double)y[j] * cpl, cy[j], wy[j]);
+        ylo = cy[j] < ylo ? cy[j] : ylo;
+        yhi = cy[j] > yhi ? cy[j] : yhi;
+    }
+    // (the barrier inside also ends the y move's reads of buf)
+    block_minmax256(ylo, yhi, redy);
+    const int ny = yhi - ylo + 3;
+    T szv[PP][9];
+    if (stage_x && nx * ny <= MAXR) {
+        // window row r = xi * ny + yi (x = lo - 1 + xi, y = ylo - 1 + yi) at
+        // buf[r * 16 + zl]; thread t loads rows (t >> 4) + 16 k, k = 0, 1, ...
+        const int nrow = nx * ny;
+        int r = t >> 4;
+        int xi = r / ny, yi = r - xi * ny;
+        const int q16 = 16 / ny, r16 = 16 - q16 * ny;
+        for (int k0 = r; k0 < nrow; k0 += 16 * U) {
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (k0 + 16 * u < nrow) {
+                    const unsigned row = (unsigned)wrapf(lo - 1 + xi, N, invN) * plane +
+                                         (unsigned)wrapf(ylo - 1 + yi, N, invN) * rs;
+                    v[u] = sz[row + qz];
+                }
+                xi += q16;
+                yi += r16;
+                if (yi >= ny) {
+                    yi -= ny;
+                    xi++;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (k0 + 16 * u < nrow) buf[(k0 + 16 * u) * 16 + zl] = v[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PP; j++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    szv[j][a * 3 + b] =
+                        buf[((cx[j] - lo + a) * ny + (cy[j] - ylo + b)) * 16 + zl];
+    } else {
+#pragma unroll
+        for (int j = 0; j < PP; j++) {
+            unsigned ox[3], oy[3];
+            tap_rows(wrapf(cx[j], N, invN), N, plane, ox);
+            tap_rows(wrapf(cy[j], N, invN), N, rs, oy);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) szv[j][a * 3 + b] = sz[ox[a] + oy[b] + qz];
+        }
+    }
+    const float Lp = (float)Lf;
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        T d = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T rr = fma(wy[j][2], szv[j][a * 3 + 2],
+                             fma(wy[j][1], szv[j][a * 3 + 1], wy[j][0] * szv[j][a * 3]));
+            d = fma(wx[j][a], rr, d);
+        }
+        const float zz = (float)((T)z - d);
+        const long long p = ((long long)(x0 + j) * N + qy) * N + qz;
+        X[p] = fwrap_float(x[j], Lp, boxlen);
+        Y[p] = fwrap_float(y[j], Lp, boxlen);
+        Z[p] = fwrap_float(zz, Lp, boxlen);
+    }
+}
+
 // analyse/src/mitos_read.c:285-320: base cell and the eight CIC weights of a
 // particle, the weights as 2^-40 fixed point in the reference's corner order
 // (000, 100, 010, 001, 110, 101, 011, 111).

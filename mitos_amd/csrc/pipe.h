@@ -350,6 +350,9 @@ struct Pipe {
     // the FP32 displacement stages its taps in LDS (MITOS_TSC_DIRECT=1: reads
     // them from the grids, the same bits; for tests)
     bool tsc_staged = true;
+    // the FP32 displacement over 8 x 8 x 16 tiles (MITOS_TSC_KERNEL=wave: the
+    // one-wave row kernel)
+    bool tsc_tile = true;
 
     // time every schedule variant of the plain passes (or reuse the choice
     // made for this device, N and precision)
@@ -495,6 +498,8 @@ struct Pipe {
         const size_t wsb = cic_stream_bytes<M>(N);
         const char* direct_env = getenv("MITOS_TSC_DIRECT");
         tsc_staged = !(direct_env && direct_env[0] == '1');
+        const char* kern_env = getenv("MITOS_TSC_KERNEL");
+        tsc_tile = !(kern_env && strcmp(kern_env, "wave") == 0);
         const char* atomic_env = getenv("MITOS_CIC_ATOMIC");
         const bool force_atomic = atomic_env && atomic_env[0] == '1';
         if (wsb && !force_atomic && hipMalloc(&cic_ws, wsb) != hipSuccess) {
@@ -731,7 +736,12 @@ struct Pipe {
                     const int nch = (NN + NT - 1) / NT;
                     const dim3 grid((unsigned)((long)NN * NN * nch));
                     const unsigned rs = (unsigned)(prs());
-                    if (STENCIL && NN % 64 == 0) {
+                    if (STENCIL && NN % 16 == 0 && tsc_tile) {
+                        const dim3 gridt((unsigned)((long)(NN / 8) * (NN / 8) * (NN / 16)));
+                        hipLaunchKernelGGL((k_displace_tile<T, 40, 608, 12>), gridt, dim3(256), 0,
+                                           s, psi(0), psi(1), psi(2), rs, NN, (T)L, spacing, L,
+                                           X, Y, Z, tsc_staged ? 1 : 0);
+                    } else if (STENCIL && NN % 64 == 0) {
                         constexpr int TX = 16, PPT = 8;
                         const dim3 gridw((unsigned)((long)NN * NN / PPT * (NN / 64)));
                         hipLaunchKernelGGL((k_displace_lattice_w<T, TX, PPT, 16, 6>), gridw,

@@ -157,24 +157,28 @@ def test_two_pass_deposit_matches_atomic(gpu, N, L, sigma8, monkeypatch):
     assert abs(float(da.astype(np.float64).mean())) < 1e-6
 
 
-@pytest.mark.parametrize("N,L,sigma8", [(128, 250.0, 0.81), (128, 64.0, 3.0)])
+@pytest.mark.parametrize("N,L,sigma8", [(128, 250.0, 0.81), (256, 500.0, 0.81),
+                                         (128, 64.0, 3.0)])
 def test_staged_displacement_matches_direct(gpu, N, L, sigma8, monkeypatch):
-    # the FP32 displacement with its taps staged in LDS and with every tap
-    # read from the grids give the same positions, bit for bit (the second
-    # case spreads the taps beyond the staging window, so both paths run)
+    # the FP32 displacement over tiles with its taps staged in LDS, the same
+    # kernel with every tap read from the grids, and the one-wave row kernel
+    # give the same positions, bit for bit (the last case spreads the taps
+    # beyond the staging windows, so the staged and direct paths both run)
     print("case staged displacement", N, L, sigma8, flush=True)
     cfg = gpu.ICConfig(N=N, boxlen=L, seed=4, bins=16, sigma8=sigma8, precision=gpu.MITOS_FP32)
-    ic = gpu.IC(cfg)
-    ic.run()
-    pa = [t.cpu().numpy() for t in ic.positions()]
-    ic.close()
-    monkeypatch.setenv("MITOS_TSC_DIRECT", "1")
-    ic = gpu.IC(cfg)
-    ic.run()
-    pb = [t.cpu().numpy() for t in ic.positions()]
-    ic.close()
-    for a, b in zip(pa, pb):
-        assert np.array_equal(a, b)
+    pos = []
+    for env in ({}, {"MITOS_TSC_DIRECT": "1"}, {"MITOS_TSC_KERNEL": "wave"}):
+        for k in ("MITOS_TSC_DIRECT", "MITOS_TSC_KERNEL"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ic = gpu.IC(cfg)
+        ic.run()
+        pos.append([t.cpu().numpy() for t in ic.positions()])
+        ic.close()
+    for other in pos[1:]:
+        for a, b in zip(pos[0], other):
+            assert np.array_equal(a, b)
 
 
 def test_fp32_pk_vs_fp64_256(gpu):
