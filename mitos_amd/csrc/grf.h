@@ -31,13 +31,35 @@ __device__ __forceinline__ Xo row_start(const NoiseJumps& J, long line) {
     return s;
 }
 
+// x * 2^s + x, written so that it stays one shift-add (v_lshl_add_u64): the
+// empty asm hides that both operands are x, so the compiler cannot fold the
+// sum back into a 64-bit multiply (two v_mad_u64_u32)
+__device__ __forceinline__ uint64_t shl_add64(uint64_t x, int s) {
+    uint64_t y = x;
+    asm("" : "+v"(y));
+    return (y << s) + x;
+}
+
+// xo_next on the device: the same draw, with s1 * 5 and r * 9 as shift-adds
+__device__ __forceinline__ uint64_t xo_next_dev(Xo& st) {
+    const uint64_t out = shl_add64(rotl64(shl_add64(st.s[1], 2), 7), 3);
+    const uint64_t t = st.s[1] << 17;
+    st.s[2] ^= st.s[0];
+    st.s[3] ^= st.s[1];
+    st.s[1] ^= st.s[2];
+    st.s[0] ^= st.s[3];
+    st.s[2] ^= t;
+    st.s[3] = rotl64(st.s[3], 45);
+    return out;
+}
+
 // src/random.c:34-49: cosine branch of Box-Muller from two 64-bit draws.
 // CR: correctly rounded log/cos (bit-exact Gaussians); else the device libm
 // (within an ulp, used when the result is rounded to float anyway).
 template <bool CR = true>
 __device__ __forceinline__ double bm_normal(Xo& gen) {
-    const uint64_t A = xo_next(gen);
-    const uint64_t B = xo_next(gen);
+    const uint64_t A = xo_next_dev(gen);
+    const uint64_t B = xo_next_dev(gen);
     const double RMax = 18446744073709551616.0;  // (double)UINT64_MAX + 1 = 2^64
     const double u = ((double)A + 0.5) / RMax;
     const double v = ((double)B + 0.5) / RMax;
@@ -45,16 +67,44 @@ __device__ __forceinline__ double bm_normal(Xo& gen) {
     return sqrt(-2 * log(u)) * cos(2 * M_PI * v);
 }
 
-// The same Box-Muller pair in float arithmetic (FP32 pipeline): the uniforms
-// are rounded to float and log, cos(2 pi v) = cospi(2 v) and sqrt are float
-// functions; the draws consumed are those of bm_normal.
+// cos(2 pi v) in float for v in [0, 1): r = v - rint(v) is exact, t = |r|
+// (or 1/2 - |r| past a quarter period, with the sign flipped) lies in
+// [0, 1/4], and cos(2 pi t) is a degree-10 even polynomial in t fitted here
+// (weighted least squares on Chebyshev nodes, iterated towards equal ripple;
+// fit error 2.2e-10, float evaluation within 1.4e-7 of cos(2 pi v))
+__device__ __forceinline__ float cos2pi_f(float v) {
+    const float r = v - rintf(v);
+    const float s = fabsf(r);
+    const bool far = s > 0.25f;
+    const float t = far ? 0.5f - s : s;
+    const float t2 = t * t;
+    float p = -24.982326173851263f;
+    p = fmaf(p, t2, 60.14403025957297f);
+    p = fmaf(p, t2, -85.45357257295981f);
+    p = fmaf(p, t2, 64.9393466552905f);
+    p = fmaf(p, t2, -19.73920854920032f);
+    p = fmaf(p, t2, 0.9999999997815415f);
+    return far ? -p : p;
+}
+
+// The Box-Muller pair of bm_normal in float arithmetic (FP32 pipeline): the
+// same draws; the 64-bit words become floats as hi * 2^32 + lo with one
+// rounding (within an ulp of the rounded word), then log, the hardware square
+// root (an ulp) and cos2pi_f.
+__device__ __forceinline__ float u64_to_float(uint64_t A) {
+    // the empty asm keeps the high word a 32-bit value, so its conversion is
+    // one v_cvt_f32_u32 rather than a 64-bit integer conversion
+    unsigned hi = (unsigned)(A >> 32);
+    asm("" : "+v"(hi));
+    return fmaf((float)hi, 4294967296.0f, (float)(unsigned)A);
+}
 __device__ __forceinline__ float bm_normal_f(Xo& gen) {
-    const uint64_t A = xo_next(gen);
-    const uint64_t B = xo_next(gen);
+    const uint64_t A = xo_next_dev(gen);
+    const uint64_t B = xo_next_dev(gen);
     const float s = 5.42101086242752217e-20f;  // 2^-64
-    const float u = ((float)A + 0.5f) * s;
-    const float v = ((float)B + 0.5f) * s;
-    return sqrtf(-2.0f * logf(u)) * cospif(2.0f * v);
+    const float u = (u64_to_float(A) + 0.5f) * s;
+    const float v = (u64_to_float(B) + 0.5f) * s;
+    return __builtin_amdgcn_sqrtf(-2.0f * logf(u)) * cos2pi_f(v);
 }
 
 // src/fft.c:29-35, evaluated without contraction so k matches the host bits
@@ -107,16 +157,17 @@ __global__ void __launch_bounds__(256)
                 const long n2 = n2xy + (long)z * z;
                 cx<T> val = mk<T>(0, 0);
                 if (n2 > 0) {
-                    double a, b;
                     if constexpr (sizeof(T) == 4) {
-                        a = (double)bm_normal_f(gen) * factor;
-                        b = (double)bm_normal_f(gen) * factor;
+                        const float ga = bm_normal_f(gen);
+                        const float gb = bm_normal_f(gen);
+                        const float fm = (float)(factor * amp(n2));
+                        val = mk<T>(ga * fm, gb * fm);
                     } else {
-                        a = bm_normal<CR>(gen) * factor;
-                        b = bm_normal<CR>(gen) * factor;
+                        const double a = bm_normal<CR>(gen) * factor;
+                        const double b = bm_normal<CR>(gen) * factor;
+                        const double m = amp(n2);
+                        val = mk<T>((T)(a * m), (T)(b * m));
                     }
-                    const double m = amp(n2);
-                    val = mk<T>((T)(a * m), (T)(b * m));
                 }
                 stage[threadIdx.x * ZC + dz] = val;
             }
