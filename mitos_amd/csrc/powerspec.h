@@ -55,12 +55,22 @@ __device__ __forceinline__ int pk_bin_of(long n2, const int* first, int bins, fl
 // Mode counts and k sums depend only on (N, boxlen, bins); they come from
 // pk_geometry.
 //
-// A wavefront takes 64 consecutive z' of one row.  Along a row n^2 grows with
-// z', so equal bins form contiguous lane runs: a segmented inclusive scan
-// (fixed order) leaves each run's total in its last lane, which adds it to
-// the wavefront's compensated accumulator of that bin in LDS (different runs,
-// different bins, no conflicts).
-template <typename T, typename XF>
+// A wavefront takes one row (x, y) at a time; lane l takes the C = ceil(H/64)
+// consecutive z' in [l C, (l + 1) C).  Along a row n^2 grows with z', so the
+// modes of one bin form one run.  Each lane sums its values run by run in
+// order: a run that starts and ends inside the lane is the whole bin of this
+// row and goes straight to the wavefront's compensated (Neumaier) accumulator
+// of that bin in LDS; the lane's first run (head) and last run (tail) may
+// continue in the neighbouring lanes.  A tail is added to the next lane's
+// head when their bins agree, then a segmented inclusive scan (fixed order)
+// over the heads leaves each run's total in its last lane, which adds it to
+// the accumulator.  Every bin of a row is added once, so lanes never write
+// the same accumulator at once, and the order of every sum is fixed.
+// The lane's z', z'^2, multiplicities and column factors are fixed across
+// rows and held in registers; the next row's values are loaded while the
+// current one is summed.  CM >= C (compile time); CM = 0: any C, no
+// prefetch, and a cross spectrum of two grids.
+template <typename T, typename XF, int CM>
 __global__ void __launch_bounds__(256)
     k_pk_partial(const cx<T>* __restrict__ a, const cx<T>* __restrict__ b, int N, long hs,
                  const int* __restrict__ edges, float cinv, int bins, double* part, XF xf) {
@@ -76,42 +86,141 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     const bool same = (a == b);
     const int H = N / 2 + 1;
-    const long nrows = (long)N * N;
-    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-    for (long row = (long)blockIdx.x * (blockDim.x >> 6) + wv; row < nrows; row += nwaves) {
-        const int x = (int)(row / N), y = (int)(row % N);
-        const int nx = freq_index(x, N), ny = freq_index(y, N);
-        const long n2xy = (long)nx * nx + (long)ny * ny;
-        const double fr = xf.row(x, y);
-        const cx<T>* ra = a + row * hs;
-        const cx<T>* rb = b + row * hs;
-        for (int z0 = 0; z0 < H; z0 += 64) {
-            const int z = z0 + lane;
-            int bin = -1;
-            double v = 0;
-            if (z < H) {
-                const long n2 = n2xy + (long)z * z;
-                if (n2 > 0) {
+    const int C = (H + 63) / 64;
+    const int z0 = lane * C, z1 = z0 + C < H ? z0 + C : H;
+    const int nrows = N * N;
+    const int nwaves = (int)gridDim.x * (int)(blockDim.x >> 6);
+    // rows of this wavefront: row0, row0 + nwaves, ...; (x, y) advance by
+    // (nwaves / N, nwaves % N) with a carry
+    const int row0 = (int)blockIdx.x * (int)(blockDim.x >> 6) + wv;
+    const int dxr = nwaves / N, dyr = nwaves - dxr * N;
+    int x = row0 / N, y = row0 - x * N;
+    // one row: v(z) summed run by run, then merged across the lanes
+    auto row_sum = [&](int xx, int yy, auto value_at) {
+        const int nx = freq_index(xx, N), ny = freq_index(yy, N);
+        const int n2xy = nx * nx + ny * ny;
+        // this lane's runs: head (hb, hsum) once a second run starts, the
+        // current run (cb, cs)
+        int hb = -1, cb = -1;
+        double hsum = 0, cs = 0;
+        auto push = [&](int n2, double v) {
+            int bn;
+            if (cb < 0) {
+                bn = pk_bin_of(n2, first, bins, cinv);
+            } else {
+                bn = cb;
+                while (bn < bins - 1 && n2 >= first[bn + 1]) bn++;
+            }
+            if (bn == cb) {
+                cs += v;
+            } else {
+                if (cb >= 0) {
+                    if (hb < 0) {
+                        hb = cb;
+                        hsum = cs;
+                    } else {
+                        kadd(acc[cb * 2 + 0], acc[cb * 2 + 1], cs);  // a run inside the lane
+                    }
+                }
+                cb = bn;
+                cs = v;
+            }
+        };
+        value_at(n2xy, push);
+        const bool multi = hb >= 0;
+        const int headb = multi ? hb : cb;
+        double head = multi ? hsum : cs;
+        const int tailb = multi ? cb : -1;
+        // the previous lane's tail joins this head when the bins agree; a
+        // tail that does not continue is a whole run
+        const int pb = __shfl_up(tailb, 1, 64);
+        const double ps = __shfl_up(multi ? cs : 0.0, 1, 64);
+        const int nb = __shfl_down(headb, 1, 64);
+        if (lane > 0 && pb >= 0 && pb == headb) head += ps;
+        if (multi && !(lane < 63 && nb == tailb)) kadd(acc[tailb * 2 + 0], acc[tailb * 2 + 1], cs);
+        // segmented inclusive scan over the heads of equal bin
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double vo = __shfl_up(head, o, 64);
+            const int bo = __shfl_up(headb, o, 64);
+            if (lane >= o && bo == headb) head += vo;
+        }
+        if (headb >= 0 && (lane == 63 || nb != headb))
+            kadd(acc[headb * 2 + 0], acc[headb * 2 + 1], head);
+    };
+    if constexpr (CM > 0) {
+        // per-lane constants of the lane's z'
+        int zsq[CM];
+        double colm[CM];  // multiplicity (src/calc_powerspec.c:75-80) / column factor
+        double colf[CM];
+#pragma unroll
+        for (int k = 0; k < CM; k++) {
+            const int z = z0 + k;
+            zsq[k] = z * z;
+            colf[k] = (k < C && z < H) ? xf.col(z) : 0.0;
+            colm[k] = (z == 0 || z == N / 2) ? 1.0 : 2.0;
+        }
+        const int nk = z1 > z0 ? z1 - z0 : 0;
+        cx<T> cur[CM];
+        const cx<T>* rp = a + (long)row0 * hs + z0;
+#pragma unroll
+        for (int k = 0; k < CM; k++)
+            if (k < nk && row0 < nrows) cur[k] = rp[k];
+        for (int row = row0; row < nrows; row += nwaves) {
+            // the next row's values and factor first
+            const int rn = row + nwaves;
+            int xn = x + dxr, yn = y + dyr;
+            if (yn >= N) {
+                yn -= N;
+                xn++;
+            }
+            cx<T> nxt[CM];
+            const cx<T>* np = a + (long)rn * hs + z0;
+#pragma unroll
+            for (int k = 0; k < CM; k++)
+                if (k < nk && rn < nrows) nxt[k] = np[k];
+            const double fr = xf.row(x, y);
+            row_sum(x, y, [&](int n2xy, auto& push) {
+#pragma unroll
+                for (int k = 0; k < CM; k++) {
+                    if (k >= nk) break;
+                    const int n2 = n2xy + zsq[k];
+                    if (n2 == 0) continue;  // the DC mode
+                    const double f = fr * colf[k];
+                    const double ur = (double)cur[k].re * f, ui = (double)cur[k].im * f;
+                    // (f u).(f u) with the factor applied to each operand
+                    push(n2, colm[k] * (ur * ur + ui * ui));
+                }
+            });
+#pragma unroll
+            for (int k = 0; k < CM; k++) cur[k] = nxt[k];
+            x = xn;
+            y = yn;
+        }
+    } else {
+        for (int row = row0; row < nrows; row += nwaves) {
+            const double fr = xf.row(x, y);
+            const cx<T>* ra = a + (long)row * hs;
+            const cx<T>* rb = b + (long)row * hs;
+            row_sum(x, y, [&](int n2xy, auto& push) {
+                for (int z = z0; z < z1; z++) {
+                    const int n2 = n2xy + z * z;
+                    if (n2 == 0) continue;  // the DC mode
                     const double f = fr * xf.col(z);
                     const cx<T> u = ra[z];
                     const cx<T> w = same ? u : rb[z];
                     const double mult = (z == 0 || z == N / 2) ? 1.0 : 2.0;
-                    bin = pk_bin_of(n2, first, bins, cinv);
                     // (f u).(f w) with the factor applied to each operand
-                    v = mult * (((double)u.re * f) * ((double)w.re * f) +
-                                ((double)u.im * f) * ((double)w.im * f));
+                    push(n2, mult * (((double)u.re * f) * ((double)w.re * f) +
+                                     ((double)u.im * f) * ((double)w.im * f)));
                 }
+            });
+            x += dxr;
+            y += dyr;
+            if (y >= N) {
+                y -= N;
+                x++;
             }
-            // segmented inclusive scan over the lane runs of equal bin
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const double vo = __shfl_up(v, o, 64);
-                const int bo = __shfl_up(bin, o, 64);
-                if (lane >= o && bo == bin) v += vo;
-            }
-            const int bnext = __shfl_down(bin, 1, 64);
-            if (bin >= 0 && (lane == 63 || bnext != bin))
-                kadd(acc[bin * 2 + 0], acc[bin * 2 + 1], v);
         }
     }
     __syncthreads();
@@ -221,8 +330,21 @@ inline int pk_run(const cx<T>* a, const cx<T>* b, int N, long hs, double boxlen,
     double* fs = dev_sums ? dev_sums : (double*)(ws + pbytes);
     long long* fo = dev_obs ? dev_obs : (long long*)(ws + pbytes + sizeof(double) * 2 * bins);
     const size_t shm = sizeof(double) * 4 * 2 * bins + sizeof(int) * (bins + 1);
-    hipLaunchKernelGGL((k_pk_partial<T, XF>), dim3(nblk), dim3(256), shm, s, a, b, N, hs, edges,
-                       pk_bin_cinv(N, bins), bins, part, xf);
+    const int C = (N / 2 + 64) / 64;
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), shm, s, a, b, N, hs, edges,
+                           pk_bin_cinv(N, bins), bins, part, xf);
+    };
+    if (a != b || C > 16)
+        launch(k_pk_partial<T, XF, 0>);
+    else if (C <= 4)
+        launch(k_pk_partial<T, XF, 4>);
+    else if (C <= 8)
+        launch(k_pk_partial<T, XF, 8>);
+    else if (C <= 12)
+        launch(k_pk_partial<T, XF, 12>);
+    else
+        launch(k_pk_partial<T, XF, 16>);
     MITOS_TRY(launch_status());
     hipLaunchKernelGGL(k_pk_final, dim3(bins), dim3(64), 0, s, part, nblk, bins,
                        geo.ksum, geo.obs, fs, fo);
