@@ -622,9 +622,10 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-// work-group minimum and maximum of an int (256 threads; `red` holds the
-// four waves' pairs)
-__device__ __forceinline__ void block_minmax256(int& lo, int& hi, int (*red)[2]) {
+// work-group minimum and maximum of an int over NW waves (`red` holds the
+// waves' pairs)
+template <int NW>
+__device__ __forceinline__ void block_minmax(int& lo, int& hi, int (*red)[2]) {
     wave_minmax(lo, hi);
     const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -633,43 +634,47 @@ __device__ __forceinline__ void block_minmax256(int& lo, int& hi, int (*red)[2])
     }
     __syncthreads();
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
+    for (int w = 0; w < NW; w++) {
         lo = red[w][0] < lo ? red[w][0] : lo;
         hi = red[w][1] > hi ? red[w][1] : hi;
     }
 }
 
-// FP32 lattice displacement over 8 x 8 x 16 lattice tiles (x, y, z), one
-// work-group of 256 threads per tile; thread t moves the four x-consecutive
-// particles (x0 + 4 (t >> 7) + j, y0 + ((t >> 4) & 7), z0 + (t & 15)).
-// The taps of a tile's y and z moves lie in small windows of the displacement
-// grids (the displacement varies little over a compact tile): after the x move
-// the tile's x taps span [xlo - 1, xhi + 1].  Those rows of S_yz psi_y (the
-// tile's 8 y and 16 z) are loaded by the whole work-group into LDS, at most
-// U loads in flight per thread, and the y move reads its taps there; after
-// the y move the rows (x window) x (y window) of S_z psi_z follow the same way.
-// A window over MAXX x rows or MAXR rows is read from the grids directly.  The
-// weights, taps and order of the sums are those of k_displace_lattice_s, so
-// the positions are the same bits.  N must be a multiple of 16.
-template <typename T, int MAXX, int MAXR, int U>
-__global__ void __launch_bounds__(256)
+// FP32 lattice displacement over TXN x 8 x TZ lattice tiles (x, y, z), one
+// work-group of TXN * 2 * TZ threads per tile; thread t moves the four
+// x-consecutive particles (x0 + 4 (t / (8 TZ)) + j, y0 + (t / TZ) % 8,
+// z0 + t % TZ).  The taps of a tile's y and z moves lie in small windows of
+// the displacement grids (the displacement varies little over a compact
+// tile): after the x move the tile's x taps span [xlo - 1, xhi + 1].  Those
+// rows of S_yz psi_y (the tile's 8 y and TZ z) are loaded by the whole
+// work-group into LDS, at most U loads in flight per thread, and the y move
+// reads its taps there; after the y move the rows (x window) x (y window) of
+// S_z psi_z follow the same way.  A window over MAXX x rows or MAXR rows is
+// read from the grids directly.  The weights, taps and order of the sums are
+// those of k_displace_lattice_s, so the positions are the same bits.
+// YFAST: tiles walk z, then y, then x (else z, x, y).  N must be a multiple
+// of TZ.
+template <typename T, int TXN, int TZ, int MAXX, int MAXR, int U, bool YFAST>
+__global__ void __launch_bounds__(TXN * 2 * TZ)
     k_displace_tile(const T* __restrict__ sx, const T* __restrict__ sy,
                     const T* __restrict__ sz, unsigned rs, int N, T Lf, float spacing,
                     double boxlen, float* __restrict__ X, float* __restrict__ Y,
                     float* __restrict__ Z, int staged) {
     static_assert(MAXX * 8 <= MAXR, "the y window fits the buffer");
-    constexpr int PP = 4;
-    __shared__ T buf[MAXR * 16];
-    __shared__ int redx[4][2], redy[4][2];
+    constexpr int PP = 4, NT = TXN * 2 * TZ, NW = NT / 64;
+    constexpr int XG = NT / (8 * TZ);  // x groups of four rows
+    constexpr int RS = NT / TZ;        // window rows per round of the work-group
+    __shared__ T buf[MAXR * TZ];
+    __shared__ int redx[NW][2], redy[NW][2];
     const int t = threadIdx.x;
-    const int zl = t & 15, yl = (t >> 4) & 7, xh = t >> 7;
-    const unsigned nzt = (unsigned)N / 16u, nxt = (unsigned)N / 8u;
-    // z tiles fastest, then x (the x halo of the next tile is still in L2)
+    const int zl = t % TZ, yl = (t / TZ) % 8, xh = t / (8 * TZ);
+    const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN, nyt = (unsigned)N / 8u;
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned zt = bid % nzt, r1 = bid / nzt;
-    const unsigned xt = r1 % nxt, yt = r1 / nxt;
-    const int qz = (int)zt * 16 + zl, qy = (int)yt * 8 + yl;
-    const int x0 = (int)xt * 8 + xh * PP;
+    const unsigned xt = YFAST ? r1 / nyt : r1 % nxt;
+    const unsigned yt = YFAST ? r1 % nyt : r1 / nxt;
+    const int qz = (int)zt * TZ + zl, qy = (int)yt * 8 + yl;
+    const int x0 = (int)xt * TXN + xh * PP;
     const float invN = 1.0f / N;
     const double cpl = (double)N / boxlen;  // as in k_displace_lattice_s
     const unsigned plane = (unsigned)N * rs;
@@ -694,33 +699,43 @@ __global__ void __launch_bounds__(256)
         lo = cx[j] < lo ? cx[j] : lo;
         hi = cx[j] > hi ? cx[j] : hi;
     }
-    block_minmax256(lo, hi, redx);
+    block_minmax<NW>(lo, hi, redx);
+    // work-group uniform: kept in scalar registers
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
     const int nx = hi - lo + 3;  // x rows lo - 1 .. hi + 1
-    const bool stage_x = staged && nx <= MAXX;
+    // (windows no wider than the grid keep the incremental wraps below to
+    // one correction each way)
+    const bool stage_x = staged && nx <= MAXX && nx <= N;
     // y move: TSC along x at (x', q_y, q_z)
     T syv[PP][3];
     if (stage_x) {
         // window row xi (x = lo - 1 + xi) of this thread's (y, z) column at
-        // buf[(xi * 8 + yl) * 16 + zl]; the thread loads rows xh, xh + 2, ...
+        // buf[(xi * 8 + yl) * TZ + zl]; the thread loads rows xh, xh + XG,
+        // ...; its wrapped x follows incrementally (every wrapped row is in
+        // the grid, so the loads past the window need no guard and are not
+        // stored)
         const unsigned col = (unsigned)qy * rs + qz;
-        for (int k0 = xh; k0 < nx; k0 += 2 * U) {
+        int xw = wrapf(lo - 1 + xh, N, invN);
+        for (int k0 = xh; k0 < nx; k0 += XG * U) {
             T v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int xi = k0 + 2 * u;
-                if (xi < nx) v[u] = sy[(unsigned)wrapf(lo - 1 + xi, N, invN) * plane + col];
+                v[u] = sy[__umul24((unsigned)xw, plane) + col];
+                xw += XG;
+                if (xw >= N) xw -= N;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int xi = k0 + 2 * u;
-                if (xi < nx) buf[(xi * 8 + yl) * 16 + zl] = v[u];
+                const int xi = k0 + XG * u;
+                if (xi < nx) buf[(xi * 8 + yl) * TZ + zl] = v[u];
             }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PP; j++)
 #pragma unroll
-            for (int a = 0; a < 3; a++) syv[j][a] = buf[((cx[j] - lo + a) * 8 + yl) * 16 + zl];
+            for (int a = 0; a < 3; a++) syv[j][a] = buf[((cx[j] - lo + a) * 8 + yl) * TZ + zl];
     } else {
 #pragma unroll
         for (int j = 0; j < PP; j++) {
@@ -745,35 +760,51 @@ __global__ void __launch_bounds__(256)
         yhi = cy[j] > yhi ? cy[j] : yhi;
     }
     // (the barrier inside also ends the y move's reads of buf)
-    block_minmax256(ylo, yhi, redy);
+    block_minmax<NW>(ylo, yhi, redy);
+    ylo = __builtin_amdgcn_readfirstlane(ylo);
+    yhi = __builtin_amdgcn_readfirstlane(yhi);
     const int ny = yhi - ylo + 3;
     T szv[PP][9];
-    if (stage_x && nx * ny <= MAXR) {
+    if (stage_x && nx * ny <= MAXR && ny <= N) {
         // window row r = xi * ny + yi (x = lo - 1 + xi, y = ylo - 1 + yi) at
-        // buf[r * 16 + zl]; thread t loads rows (t >> 4) + 16 k, k = 0, 1, ...
+        // buf[r * TZ + zl]; the TZ-lane group g = t / TZ loads the g-th of RS
+        // blocks of consecutive rows, following the wrapped row and its
+        // offset incrementally
         const int nrow = nx * ny;
-        int r = t >> 4;
+        const int per = (nrow + RS - 1) / RS;
+        int r = (t / TZ) * per;
+        const int rend = r + per < nrow ? r + per : nrow;
         int xi = r / ny, yi = r - xi * ny;
-        const int q16 = 16 / ny, r16 = 16 - q16 * ny;
-        for (int k0 = r; k0 < nrow; k0 += 16 * U) {
+        const int yw0 = wrapf(ylo - 1, N, invN);
+        const unsigned yo0 = __umul24((unsigned)yw0, rs);
+        int yw = yw0 + yi;
+        if (yw >= N) yw -= N;
+        int xw = wrapf(lo - 1 + xi, N, invN);
+        unsigned yo = __umul24((unsigned)yw, rs), xo = __umul24((unsigned)xw, plane);
+        const unsigned wrap_x = (unsigned)N * plane;
+        for (; r < rend; r += U) {
             T v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (k0 + 16 * u < nrow) {
-                    const unsigned row = (unsigned)wrapf(lo - 1 + xi, N, invN) * plane +
-                                         (unsigned)wrapf(ylo - 1 + yi, N, invN) * rs;
-                    v[u] = sz[row + qz];
+                if (r + u < rend) v[u] = sz[xo + yo + qz];
+                yi++;
+                yw++;
+                yo += rs;
+                if (yw == N) {
+                    yw = 0;
+                    yo = 0;
                 }
-                xi += q16;
-                yi += r16;
-                if (yi >= ny) {
-                    yi -= ny;
-                    xi++;
+                if (yi == ny) {
+                    yi = 0;
+                    yw = yw0;
+                    yo = yo0;
+                    xo += plane;
+                    if (xo == wrap_x) xo = 0;
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (k0 + 16 * u < nrow) buf[(k0 + 16 * u) * 16 + zl] = v[u];
+                if (r + u < rend) buf[(r + u) * TZ + zl] = v[u];
         }
         __syncthreads();
 #pragma unroll
@@ -783,7 +814,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                 for (int b = 0; b < 3; b++)
                     szv[j][a * 3 + b] =
-                        buf[((cx[j] - lo + a) * ny + (cy[j] - ylo + b)) * 16 + zl];
+                        buf[((cx[j] - lo + a) * ny + (cy[j] - ylo + b)) * TZ + zl];
     } else {
 #pragma unroll
         for (int j = 0; j < PP; j++) {
