@@ -160,7 +160,11 @@ __global__ void __launch_bounds__(256)
                     if constexpr (sizeof(T) == 4) {
                         const float ga = bm_normal_f(gen);
                         const float gb = bm_normal_f(gen);
-                        const float fm = (float)(factor * amp(n2));
+                        float fm;
+                        if constexpr (AMP::prescaled)
+                            fm = amp.f(n2);  // (float)(factor * amp) made on the host
+                        else
+                            fm = (float)(factor * amp(n2));
                         val = mk<T>(ga * fm, gb * fm);
                     } else {
                         const double a = bm_normal<CR>(gen) * factor;
@@ -208,13 +212,25 @@ __global__ void k_hermitian(cx<T>* f, long hs, int N, long batch_stride) {
 }
 
 struct AmpOne {
+    static constexpr bool prescaled = false;
     __device__ __forceinline__ double operator()(long) const { return 1.0; }
 };
 
 // amplitude looked up by the integer n^2 of the mode
 struct AmpTable {
+    static constexpr bool prescaled = false;
     const double* tab;
     __device__ __forceinline__ double operator()(long n2) const { return tab[n2]; }
+};
+
+// FP32 noise: the amplitude times the noise factor sqrt(L^3/2), rounded to
+// float on the host, so the values are those of AmpTable in half the bytes
+// (the table of a 1024^3 grid, 3.1 MB, fits one XCD's L2)
+struct AmpTableF {
+    static constexpr bool prescaled = true;
+    const float* tabf;
+    __device__ __forceinline__ float f(long n2) const { return tabf[n2]; }
+    __device__ __forceinline__ double operator()(long n2) const { return tabf[n2]; }
 };
 
 template <typename T, typename AMP, bool CR = true>

@@ -172,6 +172,37 @@ inline const double* phi1_table(int N, double boxlen, double A, double n_s, doub
     return d;
 }
 
+// The table of phi1_table times the noise factor sqrt(L^3/2) of k_grf (the
+// same double expression), rounded to float: the FP32 noise kernel's values
+inline const float* phi1_table_f(int N, double boxlen, double A, double n_s, double gamma) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, double, double, double, double>, float*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto key = std::make_tuple(dev, N, boxlen, A, n_s, gamma);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (cache.size() > 64) {
+        hipDeviceSynchronize();
+        for (auto& kv : cache) hipFree(kv.second);
+        cache.clear();
+    }
+    const std::vector<double> h = phi1_host(N, boxlen, A, n_s, gamma);
+    const double boxvol = boxlen * boxlen * boxlen;
+    const double factor = sqrt(boxvol / 2);
+    std::vector<float> f(h.size());
+    for (size_t i = 0; i < h.size(); i++) f[i] = (float)(factor * h[i]);
+    float* d = nullptr;
+    if (hipMalloc(&d, sizeof(float) * f.size()) != hipSuccess) return nullptr;
+    if (copy_sync(d, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
 // K = noise * sqrt(L^3/2) * amp[n^2] (the scaling of src/grf.c:33 and the
 // phi1 table), DC = 0; noise dense N x N x (N/2+1)
 template <typename T>
@@ -590,6 +621,16 @@ struct Pipe {
                 hipLaunchKernelGGL((k_noise_in<T>), dim3(nblk((long)N * N * (N / 2 + 1))),
                                    dim3(256), 0, s, noise_in, K, hs, N, L, amp);
                 return launch_status();
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (!amp_tab) {
+                    // an unbatched FP32 run: the prescaled float table (the
+                    // same values as the double table's product)
+                    const float* ampf = phi1_table_f(N, L, amplitude, cfg.n_s, cfg.gamma);
+                    if (!ampf) return MITOS_ERR_ALLOC;
+                    return launch_grf<T, AmpTableF, false>(K, hs, N, L, 1, 0, J, AmpTableF{ampf},
+                                                           s);
+                }
             }
             return launch_grf<T, AmpTable, sizeof(T) == 8>(K, hs, N, L, 1, 0, J, AmpTable{amp}, s);
         }));
