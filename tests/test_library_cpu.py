@@ -125,7 +125,15 @@ def test_kernel_callbacks_on_host(L):
 
 
 def test_pk_bin_table_matches_oracle(L, oracle):
-    for N, Lb, bins in ((64, 256.0, 32), (512, 1000.0, 128), (1024, 2000.0, 128)):
+    # the reference bins every mode with its own float formula
+    # (src/calc_powerspec.c:58-63); the library's table over n^2 must agree for
+    # every mode: the configurations, the ensemble's members, radix-3 sizes,
+    # odd box lengths and bin counts
+    for N, Lb, bins in ((64, 256.0, 32), (512, 1000.0, 128), (1024, 2000.0, 128),
+                        (64, 250.0, 32), (128, 500.0, 32), (256, 1000.0, 32),
+                        (24, 96.0, 12), (48, 150.0, 16), (96, 300.0, 24), (384, 750.0, 64),
+                        (32, 100.0 / 3.0, 16), (128, 64.0, 17), (256, 500.0, 200),
+                        (1536, 2000.0, 128)):
         t = np.empty(3 * (N // 2) ** 2 + 1, np.int32)
         assert L.mitos_pk_bin_table(N, Lb, bins, t.ctypes.data) == 0
         assert oracle.pk_lut_mismatches(N, Lb, bins, t) == 0
