@@ -641,8 +641,8 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi, int (*red)[2]) {
 }
 
 // FP32 lattice displacement over TXN x 8 x TZ lattice tiles (x, y, z), one
-// work-group of TXN * 2 * TZ threads per tile; thread t moves the four
-// x-consecutive particles (x0 + 4 (t / (8 TZ)) + j, y0 + (t / TZ) % 8,
+// work-group of (TXN / PP) * 8 * TZ threads per tile; thread t moves the PP
+// x-consecutive particles (x0 + PP (t / (8 TZ)) + j, y0 + (t / TZ) % 8,
 // z0 + t % TZ).  The taps of a tile's y and z moves lie in small windows of
 // the displacement grids (the displacement varies little over a compact
 // tile): after the x move the tile's x taps span [xlo - 1, xhi + 1].  Those
@@ -652,17 +652,17 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi, int (*red)[2]) {
 // S_z psi_z follow the same way.  A window over MAXX x rows or MAXR rows is
 // read from the grids directly.  The weights, taps and order of the sums are
 // those of k_displace_lattice_s, so the positions are the same bits.
-// YFAST: tiles walk z, then y, then x (else z, x, y).  N must be a multiple
-// of TZ.
-template <typename T, int TXN, int TZ, int MAXX, int MAXR, int U, bool YFAST>
-__global__ void __launch_bounds__(TXN * 2 * TZ)
+// ORDER 0: tiles walk z, then x, then y; 1: z, y, x; 2: z, then y within
+// blocks of 4, then x, then the y blocks.  N must be a multiple of TZ.
+template <typename T, int TXN, int TZ, int PP, int MAXX, int MAXR, int U, int ORDER>
+__global__ void __launch_bounds__(TXN / PP * 8 * TZ)
     k_displace_tile(const T* __restrict__ sx, const T* __restrict__ sy,
                     const T* __restrict__ sz, unsigned rs, int N, T Lf, float spacing,
                     double boxlen, float* __restrict__ X, float* __restrict__ Y,
                     float* __restrict__ Z, int staged) {
     static_assert(MAXX * 8 <= MAXR, "the y window fits the buffer");
-    constexpr int PP = 4, NT = TXN * 2 * TZ, NW = NT / 64;
-    constexpr int XG = NT / (8 * TZ);  // x groups of four rows
+    constexpr int NT = TXN / PP * 8 * TZ, NW = NT / 64;
+    constexpr int XG = NT / (8 * TZ);  // x groups of PP rows
     constexpr int RS = NT / TZ;        // window rows per round of the work-group
     __shared__ T buf[MAXR * TZ];
     __shared__ int redx[NW][2], redy[NW][2];
@@ -671,8 +671,18 @@ __global__ void __launch_bounds__(TXN * 2 * TZ)
     const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN, nyt = (unsigned)N / 8u;
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned zt = bid % nzt, r1 = bid / nzt;
-    const unsigned xt = YFAST ? r1 / nyt : r1 % nxt;
-    const unsigned yt = YFAST ? r1 % nyt : r1 / nxt;
+    unsigned xt, yt;
+    if constexpr (ORDER == 1) {
+        xt = r1 / nyt;
+        yt = r1 % nyt;
+    } else if constexpr (ORDER == 2) {
+        const unsigned r2 = r1 / 4u;
+        xt = r2 % nxt;
+        yt = (r2 / nxt) * 4u + r1 % 4u;
+    } else {
+        xt = r1 % nxt;
+        yt = r1 / nxt;
+    }
     const int qz = (int)zt * TZ + zl, qy = (int)yt * 8 + yl;
     const int x0 = (int)xt * TXN + xh * PP;
     const float invN = 1.0f / N;
