@@ -20,17 +20,6 @@ int noise_jumps(int N, const Xo& s0, NoiseJumps* out);
 // Host: state after the whole grid's draws, 4*(N*N*(N/2+1) - 1).
 int noise_advance(int N, Xo* st);
 
-// start state of z-row `line` = x*N + y.  Row 0 begins with the DC mode,
-// which takes no draw (src/grf.c:53-60); every other mode takes four.
-__device__ __forceinline__ Xo row_start(const NoiseJumps& J, long line) {
-    if (line == 0) return J.s0;
-    Xo s = J.base;
-    const unsigned long n = (unsigned long)(line - 1);
-    for (int b = 0; b < J.nbits; b++)
-        if ((n >> b) & 1UL) s = bm_apply_rows(J.tab + (size_t)b * 1024, s);
-    return s;
-}
-
 // x * 2^s + x, written so that it stays one shift-add (v_lshl_add_u64): the
 // empty asm hides that both operands are x, so the compiler cannot fold the
 // sum back into a 64-bit multiply (two v_mad_u64_u32)
@@ -51,6 +40,61 @@ __device__ __forceinline__ uint64_t xo_next_dev(Xo& st) {
     st.s[2] ^= t;
     st.s[3] = rotl64(st.s[3], 45);
     return out;
+}
+
+// Start states of the 256 z-rows line0 + t of a 256-thread work-group (line0
+// a multiple of 256), by the whole work-group.  A product M v of a jump
+// matrix and a state takes one parity per thread (output bit t = parity of
+// row t of M and v) and a ballot per wave (wave w holds word w).  The
+// work-group advances J.base to row line0 (the matrices of the bits of
+// line0 - 256 above bit 7, then J^255), then doubles: st[j + 2^k] =
+// J^(2^k) st[j] for k < 8, so thread t reads its row's state st[t].  That is
+// 255 + up to 20 products shared by 256 rows, against up to 20 per row when
+// every thread jumps on its own.  lds: 256 * 4 + 4 words.
+__device__ __forceinline__ Xo block_row_start(const NoiseJumps& J, long line0, uint64_t* lds) {
+    uint64_t(*st)[4] = reinterpret_cast<uint64_t(*)[4]>(lds);
+    uint64_t* xw = lds + 256 * 4;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    auto apply = [&](int b, const Xo& v) -> Xo {
+        const uint64_t* row = J.tab + (size_t)b * 1024 + (size_t)t * 4;
+        const uint64_t p = (row[0] & v.s[0]) ^ (row[1] & v.s[1]) ^ (row[2] & v.s[2]) ^
+                           (row[3] & v.s[3]);
+        const uint64_t w = __ballot(__builtin_popcountll(p) & 1);
+        if (lane == 0) xw[wv] = w;
+        __syncthreads();
+        const Xo r = {{xw[0], xw[1], xw[2], xw[3]}};
+        __syncthreads();
+        return r;
+    };
+    Xo v = J.base;
+    if (line0 > 0) {
+        // row line0 starts after line0 - 1 jumps: (line0 - 256) + 255
+        const unsigned long hi = (unsigned long)(line0 - 256);
+        for (int b = 8; b < J.nbits; b++)
+            if ((hi >> b) & 1UL) v = apply(b, v);
+        for (int b = 0; b < 8; b++) v = apply(b, v);
+    }
+    if (t < 4) st[0][t] = v.s[t];
+    __syncthreads();
+    for (int k = 0; k < 8 && k < J.nbits; k++) {
+        const int m = 1 << k;
+        const uint64_t* row = J.tab + (size_t)k * 1024 + (size_t)t * 4;
+        const uint64_t r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+        for (int j = 0; j < m; j++) {
+            const uint64_t p = (r0 & st[j][0]) ^ (r1 & st[j][1]) ^ (r2 & st[j][2]) ^
+                               (r3 & st[j][3]);
+            const uint64_t w = __ballot(__builtin_popcountll(p) & 1);
+            if (lane == 0) st[j + m][wv] = w;
+        }
+        __syncthreads();
+    }
+    // row 0 starts at s0 (the DC mode takes no draws); row line starts after
+    // line - 1 jumps from J.base
+    const int j = line0 > 0 ? t : t - 1;
+    Xo r = J.s0;
+    if (j >= 0) r = Xo{{st[j][0], st[j][1], st[j][2], st[j][3]}};
+    __syncthreads();  // the table's reads precede the caller's reuse of lds
+    return r;
 }
 
 // src/random.c:34-49: cosine branch of Box-Muller from two 64-bit draws.
@@ -133,7 +177,7 @@ __global__ void __launch_bounds__(256)
     const long line = line0 + threadIdx.x;
     const bool ok = line < nrows;
     const int x = (int)(line / N), y = (int)(line % N);
-    Xo gen = ok ? row_start(J, line) : J.s0;
+    Xo gen = block_row_start(J, line0, reinterpret_cast<uint64_t*>(stage));
     const double boxvol = boxlen * boxlen * boxlen;
     const double factor = sqrt(boxvol / 2);
     const int H = N / 2 + 1;

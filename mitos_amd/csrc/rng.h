@@ -94,11 +94,7 @@ __host__ __device__ __forceinline__ Xo bm_apply_rows(const uint64_t* rows, const
             const uint64_t* row = rows + (size_t)(w * 64 + i) * 4;
             const uint64_t p = (row[0] & s.s[0]) ^ (row[1] & s.s[1]) ^ (row[2] & s.s[2]) ^
                                (row[3] & s.s[3]);
-#ifdef __HIP_DEVICE_COMPILE__
-            acc |= (uint64_t)(__popcll(p) & 1) << i;
-#else
             acc |= (uint64_t)(__builtin_popcountll(p) & 1) << i;
-#endif
         }
         r.s[w] = acc;
     }

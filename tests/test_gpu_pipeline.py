@@ -242,3 +242,33 @@ def test_enqueue_on_caller_stream_then_run(gpu):
     assert np.array_equal(got_b["P"][ok], ref_b["P"][ok])
     assert np.array_equal(got_a["P"][ok], ref_a["P"][ok])
     ic.close()
+
+
+def test_two_contexts_on_two_streams(gpu):
+    """Two pipelines in flight at once on two streams share only read-only
+    tables: each realization equals a separate run bit for bit (P(k),
+    positions, density)."""
+    import torch
+    N, L, bins = 256, 500.0, 32
+    cfg_a = gpu.ICConfig(N=N, boxlen=L, seed=31, bins=bins, precision=gpu.MITOS_FP32)
+    cfg_b = gpu.ICConfig(N=N, boxlen=L, seed=32, bins=bins, precision=gpu.MITOS_FP32)
+    print("case two contexts", N, flush=True)
+    refs = []
+    for cfg in (cfg_a, cfg_b):
+        ic = gpu.IC(cfg)
+        out = ic.run(cfg)
+        refs.append((out, [t.clone() for t in ic.positions()], ic.density().clone()))
+        ic.close()
+    c1, c2 = gpu.IC(cfg_a), gpu.IC(cfg_b)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        c1.enqueue(cfg_a, stream=s1.cuda_stream)
+        c2.enqueue(cfg_b, stream=s2.cuda_stream)
+    for c, (ref, pos, dens) in zip((c1, c2), refs):
+        got = c.fetch_pk()
+        ok = ref["obs"] > 0
+        assert np.array_equal(got["P"][ok], ref["P"][ok])
+        assert all(torch.equal(a, b) for a, b in zip(c.positions(), pos))
+        assert torch.equal(c.density(), dens)
+        c.close()
