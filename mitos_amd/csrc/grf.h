@@ -29,16 +29,40 @@ __device__ __forceinline__ uint64_t shl_add64(uint64_t x, int s) {
     return (y << s) + x;
 }
 
+// 64-bit rotations and the shift by 17 on 32-bit halves with v_alignbit_b32
+// (alignbit(a, b, c) = low word of ((a:b) >> c)): two instructions each
+template <int K>
+__device__ __forceinline__ uint64_t rotl64_dev(uint64_t x) {
+    static_assert(K > 0 && K < 64 && K != 32, "rotation");
+    const unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+    unsigned nlo, nhi;
+    if constexpr (K < 32) {
+        nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - K);
+        nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - K);
+    } else {
+        nhi = __builtin_amdgcn_alignbit(lo, hi, 64 - K);
+        nlo = __builtin_amdgcn_alignbit(hi, lo, 64 - K);
+    }
+    return ((uint64_t)nhi << 32) | nlo;
+}
+template <int K>
+__device__ __forceinline__ uint64_t shl64_dev(uint64_t x) {
+    static_assert(K > 0 && K < 32, "shift");
+    const unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+    return ((uint64_t)__builtin_amdgcn_alignbit(hi, lo, 32 - K) << 32) | (lo << K);
+}
+
 // xo_next on the device: the same draw, with s1 * 5 and r * 9 as shift-adds
+// and the rotations and shift on 32-bit halves
 __device__ __forceinline__ uint64_t xo_next_dev(Xo& st) {
-    const uint64_t out = shl_add64(rotl64(shl_add64(st.s[1], 2), 7), 3);
-    const uint64_t t = st.s[1] << 17;
+    const uint64_t out = shl_add64(rotl64_dev<7>(shl_add64(st.s[1], 2)), 3);
+    const uint64_t t = shl64_dev<17>(st.s[1]);
     st.s[2] ^= st.s[0];
     st.s[3] ^= st.s[1];
     st.s[1] ^= st.s[2];
     st.s[0] ^= st.s[3];
     st.s[2] ^= t;
-    st.s[3] = rotl64(st.s[3], 45);
+    st.s[3] = rotl64_dev<45>(st.s[3]);
     return out;
 }
 
