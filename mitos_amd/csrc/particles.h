@@ -640,13 +640,13 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi, int (*red)[2]) {
     }
 }
 
-// FP32 lattice displacement over TXN x 8 x TZ lattice tiles (x, y, z), one
-// work-group of (TXN / PP) * 8 * TZ threads per tile; thread t moves the PP
-// x-consecutive particles (x0 + PP (t / (8 TZ)) + j, y0 + (t / TZ) % 8,
+// FP32 lattice displacement over TXN x TY x TZ lattice tiles (x, y, z), one
+// work-group of (TXN / PP) * TY * TZ threads per tile; thread t moves the PP
+// x-consecutive particles (x0 + PP (t / (TY TZ)) + j, y0 + (t / TZ) % TY,
 // z0 + t % TZ).  The taps of a tile's y and z moves lie in small windows of
 // the displacement grids (the displacement varies little over a compact
 // tile): after the x move the tile's x taps span [xlo - 1, xhi + 1].  Those
-// rows of S_yz psi_y (the tile's 8 y and TZ z) are loaded by the whole
+// rows of S_yz psi_y (the tile's TY y and TZ z) are loaded by the whole
 // work-group into LDS, at most U loads in flight per thread, and the y move
 // reads its taps there; after the y move the rows (x window) x (y window) of
 // S_z psi_z follow the same way.  A window over MAXX x rows or MAXR rows is
@@ -654,21 +654,21 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi, int (*red)[2]) {
 // those of k_displace_lattice_s, so the positions are the same bits.
 // ORDER 0: tiles walk z, then x, then y; 1: z, y, x; 2: z, then y within
 // blocks of 4, then x, then the y blocks.  N must be a multiple of TZ.
-template <typename T, int TXN, int TZ, int PP, int MAXX, int MAXR, int U, int ORDER>
-__global__ void __launch_bounds__(TXN / PP * 8 * TZ)
+template <typename T, int TXN, int TY, int TZ, int PP, int MAXX, int MAXR, int U, int ORDER>
+__global__ void __launch_bounds__(TXN / PP * TY * TZ)
     k_displace_tile(const T* __restrict__ sx, const T* __restrict__ sy,
                     const T* __restrict__ sz, unsigned rs, int N, T Lf, float spacing,
                     double boxlen, float* __restrict__ X, float* __restrict__ Y,
                     float* __restrict__ Z, int staged) {
-    static_assert(MAXX * 8 <= MAXR, "the y window fits the buffer");
-    constexpr int NT = TXN / PP * 8 * TZ, NW = NT / 64;
-    constexpr int XG = NT / (8 * TZ);  // x groups of PP rows
+    static_assert(MAXX * TY <= MAXR, "the y window fits the buffer");
+    constexpr int NT = TXN / PP * TY * TZ, NW = NT / 64;
+    constexpr int XG = NT / (TY * TZ);  // x groups of PP rows
     constexpr int RS = NT / TZ;        // window rows per round of the work-group
     __shared__ T buf[MAXR * TZ];
     __shared__ int redx[NW][2], redy[NW][2];
     const int t = threadIdx.x;
-    const int zl = t % TZ, yl = (t / TZ) % 8, xh = t / (8 * TZ);
-    const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN, nyt = (unsigned)N / 8u;
+    const int zl = t % TZ, yl = (t / TZ) % TY, xh = t / (TY * TZ);
+    const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN, nyt = (unsigned)N / TY;
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned zt = bid % nzt, r1 = bid / nzt;
     unsigned xt, yt;
@@ -683,7 +683,7 @@ __global__ void __launch_bounds__(TXN / PP * 8 * TZ)
         xt = r1 % nxt;
         yt = r1 / nxt;
     }
-    const int qz = (int)zt * TZ + zl, qy = (int)yt * 8 + yl;
+    const int qz = (int)zt * TZ + zl, qy = (int)yt * TY + yl;
     const int x0 = (int)xt * TXN + xh * PP;
     const float invN = 1.0f / N;
     const double cpl = (double)N / boxlen;  // as in k_displace_lattice_s
@@ -721,7 +721,7 @@ __global__ void __launch_bounds__(TXN / PP * 8 * TZ)
     T syv[PP][3];
     if (stage_x) {
         // window row xi (x = lo - 1 + xi) of this thread's (y, z) column at
-        // buf[(xi * 8 + yl) * TZ + zl]; the thread loads rows xh, xh + XG,
+        // buf[(xi * TY + yl) * TZ + zl]; the thread loads rows xh, xh + XG,
         // ...; its wrapped x follows incrementally (every wrapped row is in
         // the grid, so the loads past the window need no guard and are not
         // stored)
@@ -738,14 +738,14 @@ __global__ void __launch_bounds__(TXN / PP * 8 * TZ)
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int xi = k0 + XG * u;
-                if (xi < nx) buf[(xi * 8 + yl) * TZ + zl] = v[u];
+                if (xi < nx) buf[(xi * TY + yl) * TZ + zl] = v[u];
             }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PP; j++)
 #pragma unroll
-            for (int a = 0; a < 3; a++) syv[j][a] = buf[((cx[j] - lo + a) * 8 + yl) * TZ + zl];
+            for (int a = 0; a < 3; a++) syv[j][a] = buf[((cx[j] - lo + a) * TY + yl) * TZ + zl];
     } else {
 #pragma unroll
         for (int j = 0; j < PP; j++) {

@@ -778,17 +778,17 @@ struct Pipe {
                     const dim3 grid((unsigned)((long)NN * NN * nch));
                     const unsigned rs = (unsigned)(prs());
                     if (STENCIL && NN % 16 == 0 && tsc_tile) {
-                        auto tile = [&](auto kern, int txn, int tz, int pp) {
+                        auto tile = [&](auto kern, int txn, int ty, int tz, int pp) {
                             const dim3 gridt(
-                                (unsigned)((long)(NN / txn) * (NN / 8) * (NN / tz)));
-                            hipLaunchKernelGGL(kern, gridt, dim3(txn / pp * 8 * tz), 0, s,
+                                (unsigned)((long)(NN / txn) * (NN / ty) * (NN / tz)));
+                            hipLaunchKernelGGL(kern, gridt, dim3(txn / pp * ty * tz), 0, s,
                                                psi(0), psi(1), psi(2), rs, NN, (T)L, spacing, L,
                                                X, Y, Z, tsc_staged ? 1 : 0);
                         };
                         // 8 x 8 x 16 tiles, four particles per thread: the window
                         // limits leave about 0.2% of the tiles of a 1.95 Mpc/h cell
                         // at z = 0 amplitude to the direct reads
-                        tile(k_displace_tile<T, 8, 16, 4, 40, 608, 12, 0>, 8, 16, 4);
+                        tile(k_displace_tile<T, 8, 8, 16, 4, 40, 608, 12, 0>, 8, 8, 16, 4);
                     } else if (STENCIL && NN % 64 == 0) {
                         constexpr int TX = 16, PPT = 8;
                         const dim3 gridw((unsigned)((long)NN * NN / PPT * (NN / 64)));
