@@ -855,6 +855,193 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
     }
 }
 
+// FP64 lattice displacement in the reference's arithmetic (the 27-tap sums of
+// tsc_halo<REF>, src/grids_interp.c:267-309, for the moves of
+// src/mitos.c:827-863) over TXN x TY x TZ lattice tiles, one work-group of
+// (TXN / PP) * TY * TZ threads per tile, thread t moving the PP x-consecutive
+// particles (x0 + PP (t / (TY TZ)) + j, y0 + (t / TZ) % TY, z0 + t % TZ).
+// Each move reads its 27 taps from a window of its halo-padded displacement
+// grid staged in LDS by the whole work-group: window row r holds the ZW =
+// TZ + 2 values z0 - 1 .. z0 + TZ of grid row (xs + r / nyr, ys + r % nyr)
+// (wrapped), 32 lanes a row.  The x move's window is the tile's rows with
+// one row of margin; the y move's spans the moved particles' x taps
+// [xlo - 1, xhi + 1] times the tile's y rows with margin; the z move's spans
+// those x taps times the moved y taps.  A window over MAXR rows is not
+// staged (that move reads the grid), and a particle whose taps leave the
+// window (not the case for lattice sites) reads the grid too; the weights,
+// taps and order of the sums are those of tsc_halo<REF>, so the positions
+// are the same bits as k_displace_lattice's.  N must be a multiple of TXN,
+// TY and TZ.
+template <int TXN, int TY, int TZ, int PP, int MAXR, int U>
+__global__ void __launch_bounds__(TXN / PP * TY * TZ)
+    k_displace_tile_ref(const double* __restrict__ px, const double* __restrict__ py,
+                        const double* __restrict__ pz, unsigned rs, int N, float spacing,
+                        double boxlen, float* __restrict__ X, float* __restrict__ Y,
+                        float* __restrict__ Z) {
+#pragma clang fp contract(off)
+    constexpr int NT = TXN / PP * TY * TZ, NW = NT / 64;
+    constexpr int ZW = TZ + 2;  // z values of a window row
+    constexpr int RG = NT / 32;  // rows staged per round, 32 lanes a row
+    static_assert(ZW <= 32 && NT % 64 == 0, "a window row fits 32 lanes");
+    static_assert((TXN + 2) * (TY + 2) <= MAXR, "the x move's window fits the buffer");
+    __shared__ double buf[MAXR * ZW];
+    __shared__ int redx[NW][2], redy[NW][2];
+    const int t = threadIdx.x;
+    const int zl = t % TZ, yl = (t / TZ) % TY, xh = t / (TY * TZ);
+    const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN;
+    const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned zt = bid % nzt, r1 = bid / nzt;
+    const int z0 = (int)zt * TZ, y0 = (int)(r1 / nxt) * TY, xb = (int)(r1 % nxt) * TXN;
+    const int qz = z0 + zl, qy = y0 + yl, x0 = xb + xh * PP;
+    const float invN = 1.0f / N;
+    const double bl = boxlen;
+    const unsigned plane = (unsigned)N * rs;
+    const int rl = t & 31, rg = t >> 5;
+    // stage window rows 0 .. nrow - 1 of src, nyr rows per x, from (xs, ys)
+    auto stage = [&](const double* __restrict__ src, int nrow, int nyr, int xs, int ys) {
+        const float rcp = 1.0f / (float)nyr;
+        for (int r0 = rg; r0 < nrow; r0 += RG * U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int r = r0 + RG * u;
+                v[u] = 0;
+                if (r < nrow && rl < ZW) {
+                    int xi = (int)((float)r * rcp);
+                    int yi = r - xi * nyr;
+                    if (yi < 0) {
+                        xi--;
+                        yi += nyr;
+                    } else if (yi >= nyr) {
+                        xi++;
+                        yi -= nyr;
+                    }
+                    const unsigned o = (unsigned)wrapf(xs + xi, N, invN) * plane +
+                                       (unsigned)wrapf(ys + yi, N, invN) * rs +
+                                       (unsigned)(MITOS_PSI_HO - 1 + z0 + rl);
+                    v[u] = src[o];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int r = r0 + RG * u;
+                if (r < nrow && rl < ZW) buf[r * ZW + rl] = v[u];
+            }
+        }
+    };
+    // the 27-term sum of tsc_halo<REF> with taps from the staged window (or
+    // from the grid when they are not all in it)
+    auto tap_sum = [&](const double* __restrict__ src, bool staged, int nxr, int nyr, int xs,
+                       int ys, double Xc, double Yc, double Zc) -> double {
+#pragma clang fp contract(off)
+        int cx, cy, cz;
+        double wx[3], wy[3], wz[3];
+        tsc_axis(Xc, cx, wx);
+        tsc_axis(Yc, cy, wy);
+        tsc_axis(Zc, cz, wz);
+        cx++;
+        cy++;
+        cz++;
+        const int ax = cx - 1 - xs, ay = cy - 1 - ys, az = cz - z0;  // window index of tap 0
+        const bool in = staged && ax >= 0 && ax + 2 < nxr && ay >= 0 && ay + 2 < nyr &&
+                        az >= 0 && az + 2 < ZW;
+        if (!in) return tsc_halo<double, double, true>(src, rs, N, invN, Xc, Yc, Zc);
+        double sum = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const double* r = buf + ((ax + a) * nyr + (ay + b)) * ZW + az;
+                const double wab = wx[a] * wy[b];
+                sum += r[0] * (wab * wz[0]);
+                sum += r[1] * (wab * wz[1]);
+                sum += r[2] * (wab * wz[2]);
+            }
+        }
+        return sum;
+    };
+    float x[PP], y[PP];
+    const float z = qz * spacing;  // src/particle.c:107-109
+    const double Zc = (double)z * N / bl;
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        x[j] = (x0 + j) * spacing;
+        y[j] = qy * spacing;
+    }
+    // x move: the tile's rows with one row of margin
+    {
+        const int nyr = TY + 2, nrow = (TXN + 2) * nyr;
+        stage(px, nrow, nyr, xb - 1, y0 - 1);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PP; j++) {
+            const double d = tap_sum(px, true, TXN + 2, nyr, xb - 1, y0 - 1,
+                                     (double)x[j] * N / bl, (double)y[j] * N / bl, Zc);
+            x[j] = (float)((double)x[j] - d);
+        }
+    }
+    // y move: the moved x taps times the tile's y rows with margin
+    int lo = 1 << 30, hi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        int c;
+        double w[3];
+        tsc_axis((double)x[j] * N / bl, c, w);
+        lo = c < lo ? c : lo;
+        hi = c > hi ? c : hi;
+    }
+    // (the barrier inside also ends the x move's reads of buf)
+    block_minmax<NW>(lo, hi, redx);
+    lo = __builtin_amdgcn_readfirstlane(lo);  // first taps of the centres: x window lo .. hi + 2
+    hi = __builtin_amdgcn_readfirstlane(hi);
+    const int nx = hi - lo + 3;
+    {
+        const int nyr = TY + 2;
+        const bool st = nx <= MAXR && nx * nyr <= MAXR;
+        if (st) stage(py, nx * nyr, nyr, lo, y0 - 1);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PP; j++) {
+            const double d = tap_sum(py, st, nx, nyr, lo, y0 - 1, (double)x[j] * N / bl,
+                                     (double)y[j] * N / bl, Zc);
+            y[j] = (float)((double)y[j] - d);
+        }
+    }
+    // z move: the x taps times the moved y taps
+    int ylo = 1 << 30, yhi = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        int c;
+        double w[3];
+        tsc_axis((double)y[j] * N / bl, c, w);
+        ylo = c < ylo ? c : ylo;
+        yhi = c > yhi ? c : yhi;
+    }
+    block_minmax<NW>(ylo, yhi, redy);
+    ylo = __builtin_amdgcn_readfirstlane(ylo);
+    yhi = __builtin_amdgcn_readfirstlane(yhi);
+    const int ny = yhi - ylo + 3;
+    float zz[PP];
+    {
+        const bool st = nx <= MAXR && ny <= MAXR && nx * ny <= MAXR;
+        if (st) stage(pz, nx * ny, ny, lo, ylo);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PP; j++) {
+            const double d = tap_sum(pz, st, nx, ny, lo, ylo, (double)x[j] * N / bl,
+                                     (double)y[j] * N / bl, Zc);
+            zz[j] = (float)((double)z - d);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        const long long p = ((long long)(x0 + j) * N + qy) * N + qz;
+        X[p] = (float)fwrap_fast(x[j], boxlen);
+        Y[p] = (float)fwrap_fast(y[j], boxlen);
+        Z[p] = (float)fwrap_fast(zz[j], boxlen);
+    }
+}
+
 // analyse/src/mitos_read.c:285-320: base cell and the eight CIC weights of a
 // particle, the weights as 2^-40 fixed point in the reference's corner order
 // (000, 100, 010, 001, 110, 101, 011, 111).

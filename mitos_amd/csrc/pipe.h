@@ -802,6 +802,22 @@ struct Pipe {
                         hipLaunchKernelGGL((k_displace_lattice_s<T, NT, TX, PPT>), gridp,
                                            dim3(NT), 0, s, psi(0), psi(1), psi(2), rs, NN, nch,
                                            (T)L, spacing, L, X, Y, Z);
+                    } else if (sizeof(T) == 8 && NN % 16 == 0 && tsc_tile) {
+                      if constexpr (sizeof(T) == 8) {
+                        // the reference's 27-tap moves over 8 x 8 x 16 tiles with the
+                        // taps staged in LDS (same positions as the per-particle
+                        // kernel, which MITOS_TSC_KERNEL=wave selects)
+                        auto tile = [&](auto kern, int txn, int ty, int tz, int pp) {
+                            const dim3 gridt(
+                                (unsigned)((long)(NN / txn) * (NN / ty) * (NN / tz)));
+                            hipLaunchKernelGGL(kern, gridt, dim3(txn / pp * ty * tz), 0, s,
+                                               psi(0), psi(1), psi(2), rs, NN, spacing, L, X,
+                                               Y, Z);
+                        };
+                        // 256 window rows (37 KB of LDS, three work-groups per CU at
+                        // 168 registers) and eight rows in flight per 32 lanes
+                        tile(k_displace_tile_ref<8, 8, 16, 4, 256, 8>, 8, 8, 16, 4);
+                      }
                     } else {
                         hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,
                                            psi(0), psi(1), psi(2), rs, NN, nch, L, spacing,

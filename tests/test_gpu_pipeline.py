@@ -181,6 +181,29 @@ def test_staged_displacement_matches_direct(gpu, N, L, sigma8, monkeypatch):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("N,L,sigma8", [(64, 100.0, 0.81), (128, 250.0, 0.81),
+                                         (256, 500.0, 0.81), (128, 64.0, 3.0)])
+def test_fp64_tiled_displacement_matches_per_particle(gpu, N, L, sigma8, monkeypatch):
+    # the FP64 moves in the reference's 27-tap arithmetic with the taps staged
+    # in LDS per 8x8x16 tile, and the per-particle kernel reading every tap
+    # from the grids, give the same positions bit for bit (the last case
+    # spreads the taps beyond the windows, so both of the tile kernel's
+    # paths run)
+    print("case fp64 tiled displacement", N, L, sigma8, flush=True)
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=6, bins=16, sigma8=sigma8, precision=gpu.MITOS_FP64)
+    pos = []
+    for env in ({}, {"MITOS_TSC_KERNEL": "wave"}):
+        monkeypatch.delenv("MITOS_TSC_KERNEL", raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ic = gpu.IC(cfg)
+        ic.run()
+        pos.append([t.cpu().numpy() for t in ic.positions()])
+        ic.close()
+    for a, b in zip(pos[0], pos[1]):
+        assert np.array_equal(a, b)
+
+
 def test_fp32_pk_vs_fp64_256(gpu):
     # the FP32 pipeline at the cell size of the 1024^3 / 2000 Mpc/h workload:
     # P(k) within 1e-5 of the FP64 pipeline in every bin with more than one mode
