@@ -664,7 +664,8 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
     constexpr int NT = TXN / PP * TY * TZ, NW = NT / 64;
     constexpr int XG = NT / (TY * TZ);  // x groups of PP rows
     constexpr int RS = NT / TZ;        // window rows per round of the work-group
-    __shared__ T buf[MAXR * TZ];
+    __shared__ T buf[(MAXR + 1) * TZ];  // window rows and a spare row
+    __shared__ unsigned roff[MAXR];      // grid offsets of the z move's window rows
     __shared__ int redx[NW][2], redy[NW][2];
     const int t = threadIdx.x;
     const int zl = t % TZ, yl = (t / TZ) % TY, xh = t / (TY * TZ);
@@ -777,44 +778,38 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
     T szv[PP][9];
     if (stage_x && nx * ny <= MAXR && ny <= N) {
         // window row r = xi * ny + yi (x = lo - 1 + xi, y = ylo - 1 + yi) at
-        // buf[r * TZ + zl]; the TZ-lane group g = t / TZ loads the g-th of RS
-        // blocks of consecutive rows, following the wrapped row and its
-        // offset incrementally
+        // buf[r * TZ + zl]; the work-group first tabulates every row's
+        // wrapped grid offset in LDS, then the TZ-lane group g = t / TZ loads
+        // the g-th of RS blocks of consecutive rows (a load past the block
+        // rereads its last row and is stored to the spare row MAXR)
         const int nrow = nx * ny;
+        const float rny = 1.0f / (float)ny;
+        for (int r = t; r < nrow; r += NT) {
+            int xi = (int)((float)r * rny);
+            int yi = r - xi * ny;
+            if (yi < 0) {
+                xi--;
+                yi += ny;
+            } else if (yi >= ny) {
+                xi++;
+                yi -= ny;
+            }
+            roff[r] = __umul24((unsigned)wrapf(lo - 1 + xi, N, invN), plane) +
+                      __umul24((unsigned)wrapf(ylo - 1 + yi, N, invN), rs);
+        }
+        __syncthreads();
         const int per = (nrow + RS - 1) / RS;
-        int r = (t / TZ) * per;
-        const int rend = r + per < nrow ? r + per : nrow;
-        int xi = r / ny, yi = r - xi * ny;
-        const int yw0 = wrapf(ylo - 1, N, invN);
-        const unsigned yo0 = __umul24((unsigned)yw0, rs);
-        int yw = yw0 + yi;
-        if (yw >= N) yw -= N;
-        int xw = wrapf(lo - 1 + xi, N, invN);
-        unsigned yo = __umul24((unsigned)yw, rs), xo = __umul24((unsigned)xw, plane);
-        const unsigned wrap_x = (unsigned)N * plane;
-        for (; r < rend; r += U) {
+        const int r0 = (t / TZ) * per;
+        const int rend = r0 + per < nrow ? r0 + per : nrow;
+        for (int r = r0; r < rend; r += U) {
             T v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (r + u < rend) v[u] = sz[xo + yo + qz];
-                yi++;
-                yw++;
-                yo += rs;
-                if (yw == N) {
-                    yw = 0;
-                    yo = 0;
-                }
-                if (yi == ny) {
-                    yi = 0;
-                    yw = yw0;
-                    yo = yo0;
-                    xo += plane;
-                    if (xo == wrap_x) xo = 0;
-                }
+                const int rr = r + u < rend ? r + u : rend - 1;
+                v[u] = sz[roff[rr] + qz];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (r + u < rend) buf[(r + u) * TZ + zl] = v[u];
+            for (int u = 0; u < U; u++) buf[(r + u < rend ? r + u : MAXR) * TZ + zl] = v[u];
         }
         __syncthreads();
 #pragma unroll

@@ -788,7 +788,7 @@ struct Pipe {
                         // 8 x 8 x 16 tiles, four particles per thread: the window
                         // limits leave about 0.2% of the tiles of a 1.95 Mpc/h cell
                         // at z = 0 amplitude to the direct reads
-                        tile(k_displace_tile<T, 8, 8, 16, 4, 40, 608, 12, 0>, 8, 8, 16, 4);
+                        tile(k_displace_tile<T, 8, 8, 16, 4, 40, 576, 12, 0>, 8, 8, 16, 4);
                     } else if (STENCIL && NN % 64 == 0) {
                         constexpr int TX = 16, PPT = 8;
                         const dim3 gridw((unsigned)((long)NN * NN / PPT * (NN / 64)));
