@@ -1,10 +1,12 @@
 // Ensembles: many realizations of uneven size issued as one graph launch.
 //
-// Realizations are grouped by (N, precision).  Each group owns one pipeline
-// (its grids are reused realization after realization) and one stream; the
-// groups are forked from a root stream and joined back, so groups of
-// different sizes overlap on the GPU while realizations of one group follow
-// each other.  Every realization has its own spectrum table and its own
+// Realizations are grouped by (N, precision), and the realizations of a size
+// up to 256^3 are dealt round-robin over three groups.  Each group owns one
+// pipeline (its grids are reused realization after realization) and one
+// stream; the groups are forked from a root stream and joined back, so groups
+// overlap on the GPU while the realizations of one group follow each other.
+// Pipelines of one size share their FFT schedule, so a realization gives the
+// same bits whichever group runs it.  Every realization has its own spectrum table and its own
 // P(k) slot, so the captured sequence needs no host work between
 // realizations.  The FFT schedules of each group are autotuned first; the
 // graph is captured once, after one eager pass has made every cached table
@@ -105,6 +107,22 @@ static int ensemble_build(mitos_ensemble* e) {
             it = e->groups.end() - 1;
         }
         it->members.push_back(i);
+    }
+    {
+        // the realizations of a size up to 256^3 dealt over three pipelines
+        // (a 256^3 FP32 pipeline holds about 1.2 GB)
+        std::vector<EnsembleGroup> split;
+        for (auto& g : e->groups) {
+            const int k = std::max(1, std::min((int)g.members.size(), g.N <= 256 ? 3 : 1));
+            for (int j = 0; j < k; j++) {
+                EnsembleGroup h;
+                h.N = g.N;
+                h.precision = g.precision;
+                for (size_t m = j; m < g.members.size(); m += k) h.members.push_back(g.members[m]);
+                split.push_back(h);
+            }
+        }
+        e->groups = split;
     }
     MITOS_HIP_CHECK(hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking));
     MITOS_HIP_CHECK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
