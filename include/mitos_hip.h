@@ -426,7 +426,7 @@ int mitos_cic_overdensity_noalloc(int N, double boxlen, long long npart, const f
 #define MITOS_FP32 1
 
 struct mitos_ic_config {
-    int N;          /* [Box] GridSize; particle lattice M = N */
+    int N;          /* [Box] GridSize */
     double boxlen;  /* [Box] BoxLen (Mpc/h) */
     uint64_t seed;  /* [Random] Seed (stream of rank 0, src/mitos.c:148) */
     double n_s;     /* [Cosmology] n_s */
@@ -436,6 +436,10 @@ struct mitos_ic_config {
     int run2lpt;    /* [ParticleType_0] Run2LPT (0: Zel'dovich) */
     int bins;       /* [Read] PowerSpectrumBins */
     int precision;  /* MITOS_FP64 or MITOS_FP32 */
+    int M;          /* [ParticleType_0] CubeRootNumber: M^3 lattice particles
+                       (src/particle.c:87-129); 0 means M = N.  A context takes
+                       at most the M^3 of the config it was created with;
+                       ensembles and the noise forward/backward need M = N */
 };
 
 struct mitos_ic_result {
@@ -465,15 +469,16 @@ int mitos_ic_enqueue(mitos_ic_context *ctx, const struct mitos_ic_config *cfg, v
 /* P(k) of the last enqueued realization; this call, and the copies below,
  * first wait for the stream that realization was enqueued on */
 int mitos_ic_fetch_pk(mitos_ic_context *ctx, struct mitos_ic_result *res);
-/* copy the last run's particle positions (N^3 floats each) into caller
- * device arrays */
+/* copy the last run's particle positions (M^3 floats each, M^3 = N^3 unless
+ * the config sets M) into caller device arrays */
 int mitos_ic_positions(mitos_ic_context *ctx, float *X, float *Y, float *Z);
 /* copy the last run's overdensity (unpadded N^3) or its three displacement
  * grids (3 x N^3, x then y then z, unpadded) into a caller device buffer of
  * the context's real type (float for MITOS_FP32, double for MITOS_FP64) */
 int mitos_ic_density(mitos_ic_context *ctx, void *delta_device);
 int mitos_ic_displacements(mitos_ic_context *ctx, void *psi_device);
-/* particle ids (device, N^3) written on demand, closed form of the lattice */
+/* particle ids (device, M^3 of the last run) written on demand, closed form
+ * of the lattice */
 int mitos_ic_ids(mitos_ic_context *ctx, long long *ids_device);
 int mitos_ic_destroy(mitos_ic_context *ctx);
 

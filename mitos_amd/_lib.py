@@ -68,7 +68,8 @@ class IcConfig(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("boxlen", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("n_s", ctypes.c_double), ("sigma8", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("f1", ctypes.c_double), ("f2", ctypes.c_double),
-                ("run2lpt", ctypes.c_int), ("bins", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("run2lpt", ctypes.c_int), ("bins", ctypes.c_int), ("precision", ctypes.c_int),
+                ("M", ctypes.c_int)]
 
 
 class IcResult(ctypes.Structure):

@@ -94,6 +94,7 @@ static int ensemble_build(mitos_ensemble* e) {
     for (int i = 0; i < n; i++) {
         const mitos_ic_config& c = e->cfg[i];
         if (!size_supported(c.N) || c.bins < 1 || c.boxlen <= 0) return MITOS_ERR_ARG;
+        if (c.M != 0 && c.M != c.N) return MITOS_ERR_UNSUPPORTED;  // lattice M = N
         if (c.precision != MITOS_FP32 && c.precision != MITOS_FP64) return MITOS_ERR_ARG;
         e->bmax = std::max(e->bmax, c.bins);
         auto it = std::find_if(e->groups.begin(), e->groups.end(), [&](const EnsembleGroup& g) {

@@ -93,6 +93,7 @@ extern "C" int mitos_read_ini(const char* fname, struct mitos_ic_config* cfg) {
     std::string prec = ini_value(f, "GPU", "Precision");
     fclose(f);
     cfg->precision = (strcasecmp(prec.c_str(), "FP32") == 0) ? MITOS_FP32 : MITOS_FP64;
-    if (M != cfg->N) return MITOS_ERR_UNSUPPORTED;  // the lattice is the grid (M = N)
+    if (M < 1 || M > 2048) return MITOS_ERR_ARG;
+    cfg->M = M == cfg->N ? 0 : (int)M;  // src/particle.c:87-129 lattice of M^3 particles
     return MITOS_OK;
 }

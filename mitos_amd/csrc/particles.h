@@ -276,15 +276,16 @@ __device__ __forceinline__ M tsc_halo(const T* __restrict__ box, unsigned rs, in
     return sum;
 }
 
-// src/mitos.c:827-863 then :1072-1077 for lattice particles (src/particle.c:
-// 87-129) over halo-padded displacement grids.  One work-group per (lattice
-// row, chunk of NT sites along z); the FP64 instance (M = double) keeps the
-// reference's arithmetic, the FP32 one (M = float) moves particles in float.
+// src/mitos.c:827-863 then :1072-1077 for the particles of an Ml^3 lattice
+// (src/particle.c:87-129, spacing (float)L / Ml) over halo-padded
+// displacement grids of N^3 cells.  One work-group per (lattice row, chunk of
+// NT sites along z); the FP64 instance (M = double) keeps the reference's
+// arithmetic, the FP32 one (M = float) moves particles in float.
 template <typename T, typename M, int NT>
 __global__ void __launch_bounds__(NT)
     k_displace_lattice(const T* __restrict__ px, const T* __restrict__ py,
-                       const T* __restrict__ pz, unsigned rs, int N, int nch, double boxlen,
-                       float spacing, float* __restrict__ X,
+                       const T* __restrict__ pz, unsigned rs, int N, int Ml, int nch,
+                       double boxlen, float spacing, float* __restrict__ X,
                        float* __restrict__ Y,
                        float* __restrict__ Z) {
 #pragma clang fp contract(off)
@@ -292,8 +293,8 @@ __global__ void __launch_bounds__(NT)
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned row = bid / (unsigned)nch;
     const int z = (int)(bid - row * (unsigned)nch) * NT + threadIdx.x;
-    if (z >= N) return;
-    const int qx = (int)(row / (unsigned)N), qy = (int)(row - (unsigned)qx * N);
+    if (z >= Ml) return;
+    const int qx = (int)(row / (unsigned)Ml), qy = (int)(row - (unsigned)qx * Ml);
     float x = qx * spacing, y = qy * spacing, zz = z * spacing;  // src/particle.c:107-109
     const float invN = 1.0f / N;
     if constexpr (REF) {
@@ -321,7 +322,7 @@ __global__ void __launch_bounds__(NT)
                                       (double)zz * N / bl);
         zz = zz - d;
     }
-    const long long p = (long long)row * N + z;
+    const long long p = (long long)row * Ml + z;
     X[p] = (float)fwrap_fast(x, boxlen);
     Y[p] = (float)fwrap_fast(y, boxlen);
     Z[p] = (float)fwrap_fast(zz, boxlen);

@@ -342,6 +342,8 @@ struct Pipe {
     cx<T>* A = nullptr;  // 3 complex grids
     cx<T>* B = nullptr;  // 6 complex grids
     float *X = nullptr, *Y = nullptr, *Z = nullptr;
+    long long pcap = 0;   // particles X, Y, Z hold
+    long long npart = 0;  // particles of the last realization (M^3)
     double* pk_sums = nullptr;  // device [bins][2]
     long long* pk_obs = nullptr;
     double* pk_part = nullptr;  // per-work-group P(k) partials of this pipeline
@@ -508,8 +510,15 @@ struct Pipe {
     unsigned long long* cic() { return reinterpret_cast<unsigned long long*>(b(0)); }
     // fixed-point unit of the lattice deposit (float weights in the FP32 pipeline)
     static int cic_shift() { return sizeof(T) == 4 ? MITOS_CIC_SHIFT_F32 : MITOS_CIC_SHIFT; }
+    // overdensity scale of the fixed-point grid: 2^-shift * N^3 / npart
+    double cic_norm() const {
+        const double n3 = (double)N * N * N;
+        const double f = (npart == 0 || (double)npart == n3) ? 1.0 : n3 / (double)npart;
+        return ldexp(1.0, -cic_shift()) * f;
+    }
 
-    int create(int n) {
+    // npart_cap: the most lattice particles a realization may have (0: N^3)
+    int create(int n, long long npart_cap = 0) {
         N = n;
         const long H = N / 2 + 1;
         const long q = 128 / sizeof(cx<T>);
@@ -518,13 +527,14 @@ struct Pipe {
         // a displacement grid (rows of prs() = 2 hs reals, halo included) fits
         // in a complex grid
         if (N + MITOS_PSI_HO + 1 > 2 * hs) return MITOS_ERR_UNSUPPORTED;
-        const long long np = (long long)N * N * N;
+        const long long ng = (long long)N * N * N;
+        pcap = npart_cap > ng ? npart_cap : ng;
         MITOS_HIP_CHECK(hipMalloc(&K, sizeof(cx<T>) * csize));
         MITOS_HIP_CHECK(hipMalloc(&A, sizeof(cx<T>) * csize * 3));
         MITOS_HIP_CHECK(hipMalloc(&B, sizeof(cx<T>) * csize * 6));
-        MITOS_HIP_CHECK(hipMalloc(&X, sizeof(float) * np));
-        MITOS_HIP_CHECK(hipMalloc(&Y, sizeof(float) * np));
-        MITOS_HIP_CHECK(hipMalloc(&Z, sizeof(float) * np));
+        MITOS_HIP_CHECK(hipMalloc(&X, sizeof(float) * pcap));
+        MITOS_HIP_CHECK(hipMalloc(&Y, sizeof(float) * pcap));
+        MITOS_HIP_CHECK(hipMalloc(&Z, sizeof(float) * pcap));
         typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
         const size_t wsb = cic_stream_bytes<M>(N);
         const char* direct_env = getenv("MITOS_TSC_DIRECT");
@@ -590,9 +600,17 @@ struct Pipe {
                 const double* amp_tab = nullptr, double* pk_dst = nullptr,
                 long long* obs_dst = nullptr, const cx<T>* noise_in = nullptr,
                 int stencil = -1) {
+        // particle lattice M^3 (src/particle.c:87-129); the lattice-tile
+        // kernels (stencil form, tiled moves and deposits) need M = N
+        const int Ml = cfg.M > 0 ? cfg.M : N;
+        const long long nl = (long long)Ml * Ml * Ml;
+        if (nl > pcap) return MITOS_ERR_DIMS;
+        if (Ml != N && (noise_in || stencil > 0)) return MITOS_ERR_UNSUPPORTED;
+        npart = nl;
         // the stencil form is exact only for particles on grid nodes
         const bool STENCIL =
-            stencil < 0 ? (sizeof(T) == 4 && lattice_on_nodes(N, cfg.boxlen)) : stencil != 0;
+            Ml == N && (stencil < 0 ? (sizeof(T) == 4 && lattice_on_nodes(N, cfg.boxlen))
+                                    : stencil != 0);
         psi_stencilled = STENCIL;
         const double L = cfg.boxlen;
         const double dk = 2 * M_PI / L;
@@ -747,12 +765,12 @@ struct Pipe {
                             [&] { return psi_c2r(b(2), psi(2), OpIkz{dk}); }));
             mark(4);
             // 9-10. lattice + sequential TSC displacement + wrap + CIC deposit
-            const float spacing = (float)L / NN;
+            const float spacing = (float)L / Ml;  // src/particle.c:102-103
             // the two-pass deposit of the lattice particles stores every
             // grid cell, the one-pass and the adjoint-mode deposits add to a
             // cleared grid (far-particle lists use K, free after combine_x)
             typedef typename std::conditional<sizeof(T) == 4, float, double>::type M;
-            const bool two_pass = !noise_in && cic_ws && NN % 32 == 0 &&
+            const bool two_pass = !noise_in && cic_ws && NN % 32 == 0 && Ml == NN &&
                                   cic_list_bytes<M>(NN) <= sizeof(cx<T>) * (size_t)csize;
             if (!two_pass)
                 MITOS_TRY(timed(s, "cic_clear", 8.0 * np, [&] {
@@ -772,12 +790,20 @@ struct Pipe {
                 }));
             } else {
                 // reads 3 displacement grids, writes 3 float coordinates
-                MITOS_TRY(timed(s, "displace_tsc", 3 * Rg + 12.0 * np, [&] {
+                MITOS_TRY(timed(s, "displace_tsc", 3 * Rg + 12.0 * nl, [&] {
                     constexpr int NT = NN >= 256 ? 256 : 64;
                     const int nch = (NN + NT - 1) / NT;
                     const dim3 grid((unsigned)((long)NN * NN * nch));
                     const unsigned rs = (unsigned)(prs());
-                    if (STENCIL && NN % 16 == 0 && tsc_tile) {
+                    if (Ml != NN) {
+                        // an M^3 lattice off the grid nodes: the 27-tap moves
+                        // at each particle's position
+                        const int nchm = (Ml + NT - 1) / NT;
+                        hipLaunchKernelGGL((k_displace_lattice<T, M, NT>),
+                                           dim3((unsigned)((long)Ml * Ml * nchm)), dim3(NT), 0, s,
+                                           psi(0), psi(1), psi(2), rs, NN, Ml, nchm, L, spacing,
+                                           X, Y, Z);
+                    } else if (STENCIL && NN % 16 == 0 && tsc_tile) {
                         auto tile = [&](auto kern, int txn, int ty, int tz, int pp) {
                             const dim3 gridt(
                                 (unsigned)((long)(NN / txn) * (NN / ty) * (NN / tz)));
@@ -820,14 +846,14 @@ struct Pipe {
                       }
                     } else {
                         hipLaunchKernelGGL((k_displace_lattice<T, M, NT>), grid, dim3(NT), 0, s,
-                                           psi(0), psi(1), psi(2), rs, NN, nch, L, spacing,
+                                           psi(0), psi(1), psi(2), rs, NN, NN, nch, L, spacing,
                                            X, Y, Z);
                     }
                     return launch_status();
                 }));
                 // reads 3 float coordinates, writes the grid (two passes: the
                 // windows through the workspace, then summed into the grid)
-                MITOS_TRY(timed(s, "cic_deposit", two_pass ? 12.0 * np : 20.0 * np, [&] {
+                MITOS_TRY(timed(s, "cic_deposit", two_pass ? 12.0 * np : 12.0 * nl + 8.0 * np, [&] {
                     if constexpr (sizeof(M) == 4) {
                         if (two_pass) {
                             MITOS_TRY((cic_lattice_launch<M, true>(X, Y, Z, NN, L, cic(), cic_ws,
@@ -835,12 +861,12 @@ struct Pipe {
                             return launch_status();
                         }
                     }
-                    if (NN % 32 == 0) {
+                    if (NN % 32 == 0 && Ml == NN) {
                         MITOS_TRY((cic_lattice_launch<M, false>(X, Y, Z, NN, L, cic(), nullptr,
                                                                 nullptr, s)));
                         return launch_status();
                     }
-                    hipLaunchKernelGGL(k_cic<M>, dim3(nblk(np)), dim3(256), 0, s, X, Y, Z, np, NN, L,
+                    hipLaunchKernelGGL(k_cic<M>, dim3(nblk(nl)), dim3(256), 0, s, X, Y, Z, nl, NN, L,
                                        cic());
                     return launch_status();
                 }));
@@ -859,7 +885,8 @@ struct Pipe {
             }
             mark(6);
             // 11. density r2c (overdensity on load), 12. P(k) with CIC window
-            CicLoader<T> ld{cic(), (long)NN, ldexp(1.0, -cic_shift())};
+            // delta = sum_w * N^3 / npart - 1 (analyse/src/mitos_read.c:363-376)
+            CicLoader<T> ld{cic(), (long)NN, cic_norm()};
             MITOS_TRY(timed(s, "density_z_r2c", 8.0 * np + G, [&] {
                 return zr2c_pass_tuned<T, NN>(sched.v[TUNE_Z_R2C], ld, K, hs, 1, 0, OpNone(), s);
             }));

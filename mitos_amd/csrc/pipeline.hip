@@ -36,6 +36,7 @@ extern "C" int mitos_ic_create(const struct mitos_ic_config* cfg, mitos_ic_conte
     if (!size_supported(cfg->N)) return set_last_error(MITOS_ERR_UNSUPPORTED);
     if (cfg->precision != MITOS_FP32 && cfg->precision != MITOS_FP64)
         return set_last_error(MITOS_ERR_ARG);
+    if (cfg->M < 0 || cfg->M > 2048) return set_last_error(MITOS_ERR_ARG);
     pool_trim();  // cached drop-in blocks give way to the pipeline's grids
     mitos_ic_context* c = new mitos_ic_context();
     c->N = cfg->N;
@@ -55,14 +56,16 @@ extern "C" int mitos_ic_create(const struct mitos_ic_config* cfg, mitos_ic_conte
         delete c;
         return set_last_error(MITOS_ERR_DEVICE);
     }
+    // the particle arrays hold the config's M^3 lattice (at least N^3)
+    const long long Ml = cfg->M > 0 ? cfg->M : cfg->N;
     int rc;
     if (cfg->precision == MITOS_FP32) {
         c->pf = new Pipe<float>();
-        rc = c->pf->create(cfg->N);
+        rc = c->pf->create(cfg->N, Ml * Ml * Ml);
         if (rc == MITOS_OK) rc = c->pf->autotune(c->stream);
     } else {
         c->pd = new Pipe<double>();
-        rc = c->pd->create(cfg->N);
+        rc = c->pd->create(cfg->N, Ml * Ml * Ml);
         if (rc == MITOS_OK) rc = c->pd->autotune(c->stream);
     }
     if (rc != MITOS_OK) {
@@ -112,8 +115,14 @@ static int order_after_last(mitos_ic_context* c, hipStream_t s) {
 static int check_cfg(mitos_ic_context* c, const struct mitos_ic_config* cfg) {
     if (!c || !cfg) return MITOS_ERR_ARG;
     if (cfg->N != c->N || cfg->precision != c->precision) return MITOS_ERR_DIMS;
-    if (cfg->bins < 1 || cfg->boxlen <= 0) return MITOS_ERR_ARG;
+    if (cfg->bins < 1 || cfg->boxlen <= 0 || cfg->M < 0 || cfg->M > 2048) return MITOS_ERR_ARG;
     return MITOS_OK;
+}
+
+// lattice particles of the last realization (N^3 before the first)
+static long long ctx_npart(const mitos_ic_context* c) {
+    const long long n = c->pf ? c->pf->npart : c->pd->npart;
+    return n > 0 ? n : (long long)c->N * c->N * c->N;
 }
 
 namespace mitos {
@@ -238,7 +247,7 @@ extern "C" int mitos_ic_profile_entry(mitos_ic_context* c, int i, const char** n
 extern "C" int mitos_ic_positions(mitos_ic_context* c, float* X, float* Y, float* Z) {
     if (!c || !X || !Y || !Z) return set_last_error(MITOS_ERR_ARG);
     MITOS_TRY(set_last_error(wait_caller_stream(c)));
-    const size_t bytes = sizeof(float) * (size_t)c->N * c->N * c->N;
+    const size_t bytes = sizeof(float) * (size_t)ctx_npart(c);
     float* src[3] = {c->pf ? c->pf->X : c->pd->X, c->pf ? c->pf->Y : c->pd->Y,
                      c->pf ? c->pf->Z : c->pd->Z};
     float* dst[3] = {X, Y, Z};
@@ -328,7 +337,7 @@ extern "C" int mitos_ic_density(mitos_ic_context* c, void* delta) {
     MITOS_TRY(set_last_error(wait_caller_stream(c)));
     const long N = c->N;
     const long n = N * N * N;
-    const double norm = ldexp(1.0, -(c->pf ? Pipe<float>::cic_shift() : Pipe<double>::cic_shift()));
+    const double norm = c->pf ? c->pf->cic_norm() : c->pd->cic_norm();
     if (c->pf)
         hipLaunchKernelGGL((k_cic_to_delta<float>), dim3(nblk(n)), dim3(256), 0, c->stream,
                            c->pf->cic(), (float*)delta, n, N, N, norm);
@@ -342,7 +351,7 @@ extern "C" int mitos_ic_density(mitos_ic_context* c, void* delta) {
 
 extern "C" int mitos_ic_ids(mitos_ic_context* c, long long* ids) {
     if (!c || !ids) return set_last_error(MITOS_ERR_ARG);
-    const long long n = (long long)c->N * c->N * c->N;
+    const long long n = ctx_npart(c);
     hipLaunchKernelGGL(k_ids, dim3(nblk(n)), dim3(256), 0, c->stream, n, 1LL, ids);
     MITOS_TRY(set_last_error(launch_status()));
     MITOS_HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -56,6 +56,9 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
     const int N = cfg->N;
     const double L = cfg->boxlen;
     const long long np = (long long)N * N * N;
+    // particle lattice M^3 (src/particle.c:87-129; M = N unless cfg->M is set)
+    const int Ml = cfg->M > 0 ? cfg->M : N;
+    const long long npart = (long long)Ml * Ml * Ml;
     double t0 = now(), t1;
     float stage[8] = {0};
     int rc = MITOS_OK;
@@ -134,17 +137,17 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
     float *dX = X, *dY = Y, *dZ = Z;
     std::vector<void*> tmp;
     if (!dX || !dY || !dZ) {
-        dX = (float*)mitos::pool_alloc(sizeof(float) * np);
-        dY = (float*)mitos::pool_alloc(sizeof(float) * np);
-        dZ = (float*)mitos::pool_alloc(sizeof(float) * np);
+        dX = (float*)mitos::pool_alloc(sizeof(float) * npart);
+        dY = (float*)mitos::pool_alloc(sizeof(float) * npart);
+        dZ = (float*)mitos::pool_alloc(sizeof(float) * npart);
         tmp = {dX, dY, dZ};
         if (!dX || !dY || !dZ) {
             for (void* p : tmp) mitos::pool_free(p);
             return MITOS_ERR_ALLOC;
         }
     }
-    rc = mitos_lattice(N, L, 1, dX, dY, dZ, NULL);
-    if (rc == MITOS_OK) rc = mitos_displace_tsc_dg(psi[0], psi[1], psi[2], np, dX, dY, dZ);
+    rc = mitos_lattice(Ml, L, 1, dX, dY, dZ, NULL);
+    if (rc == MITOS_OK) rc = mitos_displace_tsc_dg(psi[0], psi[1], psi[2], npart, dX, dY, dZ);
     for (int i = 0; i < 3; i++) G.drop(psi[i]);
     t1 = now();
     stage[4] = (float)(1e3 * (t1 - t0));
@@ -158,7 +161,7 @@ extern "C" int mitos_ic_run_reference_order(const struct mitos_ic_config* cfg,
         fbox = (fftw_complex*)mitos::pool_alloc(sizeof(double) * 2 * N * (long)N * (N / 2 + 1));
         if (!box || !fbox) rc = MITOS_ERR_ALLOC;
     }
-    if (rc == MITOS_OK) rc = mitos_cic_overdensity(N, L, np, dX, dY, dZ, box);
+    if (rc == MITOS_OK) rc = mitos_cic_overdensity(N, L, npart, dX, dY, dZ, box);
     t1 = now();
     stage[5] = (float)(1e3 * (t1 - t0));
     t0 = t1;

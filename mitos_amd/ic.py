@@ -223,17 +223,22 @@ class ICConfig:
     run2lpt: int = 1
     bins: int = 32
     precision: int = MITOS_FP64
+    M: int = 0  # particle lattice M^3 (CubeRootNumber); 0 means M = N
+
+    @property
+    def npart(self) -> int:
+        return (self.M or self.N) ** 3
 
     def c(self) -> _lib.IcConfig:
         return _lib.IcConfig(self.N, self.boxlen, self.seed, self.n_s, self.sigma8, self.gamma,
-                             self.f1, self.f2, self.run2lpt, self.bins, self.precision)
+                             self.f1, self.f2, self.run2lpt, self.bins, self.precision, self.M)
 
     @staticmethod
     def from_ini(path: str) -> "ICConfig":
         c = _lib.IcConfig()
         check(lib().mitos_read_ini(path.encode(), ctypes.byref(c)), "mitos_read_ini")
         return ICConfig(c.N, c.boxlen, c.seed, c.n_s, c.sigma8, c.gamma, c.f1, c.f2, c.run2lpt,
-                        c.bins, c.precision)
+                        c.bins, c.precision, c.M)
 
 
 class IC:
@@ -255,12 +260,14 @@ class IC:
                             obs.ctypes.data_as(_lib.c_ll_p), 0.0)
         check(lib().mitos_ic_run(self.h, ctypes.byref(cfg.c()), ctypes.byref(res)),
               "mitos_ic_run")
+        self._npart = cfg.npart
         return {"k": k, "P": P, "obs": obs, "amplitude": res.amplitude,
                 "stage_ms": list(res.stage_ms)}
 
     def enqueue(self, cfg: ICConfig | None = None, stream: int | None = None) -> None:
         cfg = cfg or self.cfg
         check(lib().mitos_ic_enqueue(self.h, ctypes.byref(cfg.c()), stream), "mitos_ic_enqueue")
+        self._npart = cfg.npart
 
     def fetch_pk(self) -> dict:
         """P(k) of the last enqueued realization (waits for its stream)."""
@@ -354,7 +361,7 @@ class IC:
 
     def positions(self):
         import torch
-        n = self.cfg.N ** 3
+        n = getattr(self, "_npart", self.cfg.npart)
         X = torch.empty(n, dtype=torch.float32, device="cuda")
         Y = torch.empty_like(X)
         Z = torch.empty_like(X)
@@ -384,7 +391,8 @@ class IC:
 
     def ids(self):
         import torch
-        out = torch.empty(self.cfg.N ** 3, dtype=torch.int64, device="cuda")
+        out = torch.empty(getattr(self, "_npart", self.cfg.npart), dtype=torch.int64,
+                          device="cuda")
         _sync()
         check(lib().mitos_ic_ids(self.h, _ptr(out)), "mitos_ic_ids")
         return out
@@ -420,7 +428,7 @@ def run_reference_order(cfg: ICConfig, positions: bool = False) -> dict:
     ptrs = (None, None, None)
     if positions:
         import torch
-        n = cfg.N ** 3
+        n = cfg.npart
         X = torch.empty(n, dtype=torch.float32, device="cuda")
         Y = torch.empty_like(X)
         Z = torch.empty_like(X)

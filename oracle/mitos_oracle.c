@@ -1101,6 +1101,8 @@ int orc_ic_run(const orc_ic_config *cfg, orc_ic_outputs *out) {
     const size_t half = (size_t)N * N * (N / 2 + 1);
     const size_t padded = (size_t)N * N * (N + 2);
     const long long np = (long long)N * N * N;
+    const int M = cfg->M > 0 ? cfg->M : N; /* particle lattice, src/particle.c:87-129 */
+    const long long npart = (long long)M * M * M;
     double t0 = orc_now(), t1;
     for (int i = 0; i < 8; i++) out->stage_seconds[i] = 0;
 
@@ -1159,18 +1161,18 @@ int orc_ic_run(const orc_ic_config *cfg, orc_ic_outputs *out) {
     t1 = orc_now(); out->stage_seconds[3] = t1 - t0; t0 = t1;
 
     /* src/particle.c:87-129; src/mitos.c:827-863, :1072-1077 */
-    float *X = out->X ? out->X : (float *)malloc(sizeof(float) * np);
-    float *Y = out->Y ? out->Y : (float *)malloc(sizeof(float) * np);
-    float *Z = out->Z ? out->Z : (float *)malloc(sizeof(float) * np);
-    orc_lattice(N, L, 1, X, Y, Z, out->id);
-    orc_displace_tsc(N, L, psi, psi + padded, psi + 2 * padded, np, X, Y, Z);
+    float *X = out->X ? out->X : (float *)malloc(sizeof(float) * npart);
+    float *Y = out->Y ? out->Y : (float *)malloc(sizeof(float) * npart);
+    float *Z = out->Z ? out->Z : (float *)malloc(sizeof(float) * npart);
+    orc_lattice(M, L, 1, X, Y, Z, out->id);
+    orc_displace_tsc(N, L, psi, psi + padded, psi + 2 * padded, npart, X, Y, Z);
     if (!out->psi) free(psi);
     t1 = orc_now(); out->stage_seconds[4] = t1 - t0; t0 = t1;
 
     /* analyse/src/mitos_read.c:278-376 */
     double *dens = out->density ? out->density : (double *)calloc(np, sizeof(double));
     if (out->density) memset(dens, 0, sizeof(double) * np);
-    double total = orc_cic(N, L, np, X, Y, Z, 1.0, dens);
+    double total = orc_cic(N, L, npart, X, Y, Z, 1.0, dens);
     orc_overdensity(N, L, total, dens);
     if (!out->X) free(X);
     if (!out->Y) free(Y);

@@ -159,7 +159,7 @@ void orc_powerspec(int N, double boxlen, const double complex *b1, const double 
  * src/mitos.c:148,205-228,300,352-423,768-863,1072-1077 and
  * analyse/src/mitos_read.c:278-434.  Outputs are optional (NULL = skip). */
 typedef struct {
-    int N;             /* grid size, also particle lattice size M = N */
+    int N;             /* grid size */
     double boxlen;
     uint64_t seed;
     double n_s;
@@ -168,12 +168,13 @@ typedef struct {
     double f1, f2;     /* 2LPT factors (1, -3/7) */
     int run2lpt;
     int bins;
+    int M;             /* particle lattice M^3 (src/particle.c:87-129); 0: M = N */
 } orc_ic_config;
 
 typedef struct {
     double *psi;       /* 3 * N*N*(N+2) displacement grids x,y,z (padded) */
-    float *X, *Y, *Z;  /* N^3 particle positions after wrap */
-    long long *id;     /* N^3 particle ids */
+    float *X, *Y, *Z;  /* M^3 particle positions after wrap */
+    long long *id;     /* M^3 particle ids */
     double *density;   /* N^3 overdensity (unpadded) */
     double *k_in_bins, *power_in_bins;
     int *obs_in_bins;

@@ -48,7 +48,7 @@ class OrcIcConfig(ctypes.Structure):
     _fields_ = [("N", ctypes.c_int), ("boxlen", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("n_s", ctypes.c_double), ("sigma8", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("f1", ctypes.c_double), ("f2", ctypes.c_double),
-                ("run2lpt", ctypes.c_int), ("bins", ctypes.c_int)]
+                ("run2lpt", ctypes.c_int), ("bins", ctypes.c_int), ("M", ctypes.c_int)]
 
 
 class OrcIcOutputs(ctypes.Structure):
@@ -418,7 +418,7 @@ def ic_run(N: int, boxlen: float, seed: int, n_s: float = 0.965, sigma8: float =
            gamma: float = 0.21, bins: int = 32, run2lpt: bool = True, f1: float = 1.0,
            f2: float = -3.0 / 7.0, want_psi: bool = False, want_density: bool = False,
            want_delta_k: bool = False, threads: int | None = None,
-           fft_threads: int | None = None):
+           fft_threads: int | None = None, M: int = 0):
     """Reference-order FP64 pipeline; returns a dict of numpy outputs.
     threads: OpenMP threads of the loops; fft_threads: of the FFTs alone
     (1 = serial, as the reference's FFTW plans; None = as the loops)."""
@@ -426,8 +426,8 @@ def ic_run(N: int, boxlen: float, seed: int, n_s: float = 0.965, sigma8: float =
     if threads is not None:
         L.orc_set_threads(threads)
     L.orc_set_fft_threads(fft_threads or 0)
-    n = N ** 3
-    cfg = OrcIcConfig(N, boxlen, seed, n_s, sigma8, gamma, f1, f2, int(run2lpt), bins)
+    n = (M or N) ** 3  # lattice particles
+    cfg = OrcIcConfig(N, boxlen, seed, n_s, sigma8, gamma, f1, f2, int(run2lpt), bins, M)
     res = {"X": np.empty(n, np.float32), "Y": np.empty(n, np.float32),
            "Z": np.empty(n, np.float32), "id": np.empty(n, np.int64),
            "k": np.empty(bins), "P": np.empty(bins), "obs": np.empty(bins, np.int32)}

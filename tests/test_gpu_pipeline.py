@@ -101,6 +101,49 @@ def test_reference_order_fp64_vs_oracle(gpu, oracle):
         assert np.all(d <= ulp)
 
 
+@pytest.mark.parametrize("N,M,L,prec", [
+    (32, 24, 128.0, "fp64"), (32, 48, 100.0, "fp64"), (64, 40, 256.0, "fp64"),
+    (32, 48, 100.0, "fp32"), (64, 40, 256.0, "fp32")])
+def test_particle_lattice_other_than_grid(gpu, oracle, N, M, L, prec):
+    # [ParticleType_0] CubeRootNumber M != GridSize N (src/particle.c:87-129):
+    # M^3 particles on the lattice of spacing (float)L / M, moved by the
+    # 27-tap TSC at their positions and deposited with CIC, the overdensity
+    # normalised by the M^3 particles (analyse/src/mitos_read.c:363-376)
+    print("case lattice", N, M, L, prec, flush=True)
+    seed, bins = 11, 16
+    ref = oracle.ic_run(N, L, seed, bins=bins, want_density=True, M=M)
+    p = gpu.MITOS_FP64 if prec == "fp64" else gpu.MITOS_FP32
+    cfg = gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=p, M=M)
+    ic = gpu.IC(cfg)
+    out = ic.run()
+    assert np.array_equal(out["obs"], ref["obs"])
+    err = _pk_rel(out["P"], ref["P"], ref["obs"])
+    X, Y, Z = (t.cpu().numpy() for t in ic.positions())
+    assert X.size == M ** 3
+    pe = max(_pos_err(X, ref["X"], L), _pos_err(Y, ref["Y"], L), _pos_err(Z, ref["Z"], L))
+    dens = ic.density().cpu().numpy()
+    de = np.linalg.norm(dens - ref["density"]) / np.linalg.norm(ref["density"])
+    print(f"  P(k) max rel err {err.max():.2e}, max position diff {pe:.2e}, "
+          f"density rel err {de:.2e}", flush=True)
+    if prec == "fp64":
+        assert err.max() < 1e-10
+        assert np.array_equal(X, ref["X"]) and np.array_equal(Y, ref["Y"])
+        assert np.array_equal(Z, ref["Z"])
+        assert de < 1e-11
+    else:
+        assert err.max() < 5e-6
+        assert pe < 1e-4 * L / N
+        assert de < 1e-5
+    assert np.array_equal(ic.ids().cpu().numpy(), ref["id"])
+    ic.close()
+    if prec == "fp64":
+        # the reference call order through the drop-in functions
+        rp = gpu.run_reference_order(cfg, positions=True)
+        assert np.array_equal(rp["obs"], ref["obs"])
+        assert _pk_rel(rp["P"], ref["P"], ref["obs"]).max() < 1e-10
+        assert rp["X"].numel() == M ** 3
+
+
 def test_fused_repeatable(gpu):
     cfg = gpu.ICConfig(N=32, boxlen=100.0, seed=5, bins=16, precision=gpu.MITOS_FP32)
     ic = gpu.IC(cfg)

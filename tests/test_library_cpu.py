@@ -147,7 +147,7 @@ def test_sigma8_amplitude_matches_oracle(L, oracle):
 
 
 def test_read_ini(L, tmp_path):
-    from mitos_amd import _lib
+    from mitos_amd import _lib, ic
     p = tmp_path / "ic.ini"
     p.write_text("""; parameters of the IC path
 [Random]
@@ -173,8 +173,11 @@ Precision = FP32
     assert (c.N, c.boxlen, c.seed, c.n_s, c.sigma8, c.gamma, c.run2lpt, c.bins, c.precision) == \
         (64, 256.0, 42, 0.965, 0.81, 0.21, 1, 32, _lib.MITOS_FP32)
     assert (c.f1, c.f2) == (1.0, -3.0 / 7.0)
+    assert c.M == 0  # CubeRootNumber = GridSize: M = N
     p.write_text("[Box]\nGridSize = 64\n[ParticleType_0]\nCubeRootNumber = 32\n")
-    assert L.mitos_read_ini(str(p).encode(), ctypes.byref(c)) == 4  # M != N unsupported
+    assert L.mitos_read_ini(str(p).encode(), ctypes.byref(c)) == 0
+    assert (c.N, c.M) == (64, 32)
+    assert ic.ICConfig.from_ini(str(p)).npart == 32 ** 3
     assert L.mitos_read_ini(b"/nonexistent.ini", ctypes.byref(c)) == 5
 
 
