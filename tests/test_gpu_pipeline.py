@@ -135,7 +135,19 @@ def test_particle_lattice_other_than_grid(gpu, oracle, N, M, L, prec):
         assert pe < 1e-4 * L / N
         assert de < 1e-5
     assert np.array_equal(ic.ids().cpu().numpy(), ref["id"])
+    if M < N:
+        # more particles than the context was made for: MITOS_ERR_DIMS
+        from mitos_amd._lib import MitosError
+        with pytest.raises(MitosError) as e:
+            ic.run(gpu.ICConfig(N=N, boxlen=L, seed=seed, bins=bins, precision=p, M=N + 8))
+        assert e.value.code == 1
     ic.close()
+    if prec == "fp64" and M < N:
+        # ensembles keep the lattice on the grid
+        from mitos_amd._lib import MitosError
+        with pytest.raises(MitosError) as e:
+            gpu.Ensemble([cfg])
+        assert e.value.code == 4
     if prec == "fp64":
         # the reference call order through the drop-in functions
         rp = gpu.run_reference_order(cfg, positions=True)
