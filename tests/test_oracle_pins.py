@@ -399,6 +399,27 @@ def test_cic_unperturbed_lattice_is_uniform(oracle):
     assert np.abs(box).max() < 1e-12
 
 
+def test_pipeline_particle_lattice_m(oracle):
+    # CubeRootNumber M (src/particle.c:87-129) in the oracle's pipeline:
+    # M = N is the default run bit for bit; an M = 2N lattice barely moved
+    # (sigma8 = 1e-9) deposits a uniform density on the N grid, which fixes
+    # the lattice spacing L / M and the normalisation by the M^3 particles
+    # (an N^3 normalisation would give delta = 7 everywhere); an M = N / 2
+    # lattice conserves mass
+    N, L = 16, 64.0
+    a = oracle.ic_run(N, L, 5, bins=8, want_density=True)
+    b = oracle.ic_run(N, L, 5, bins=8, want_density=True, M=N)
+    for key in ("P", "X", "Y", "Z", "density"):
+        assert np.array_equal(a[key], b[key])
+    c = oracle.ic_run(N, L, 5, bins=8, want_density=True, sigma8=1e-9, M=2 * N)
+    assert c["X"].size == (2 * N) ** 3
+    assert np.abs(c["density"]).max() < 1e-6
+    d = oracle.ic_run(N, L, 5, bins=8, want_density=True, M=N // 2)
+    assert d["X"].size == (N // 2) ** 3
+    assert abs(d["density"].mean()) < 1e-12
+    assert d["density"].min() == -1.0  # cells no particle reaches
+
+
 # ---- a20: P(k) binning, counts and Parseval (src/calc_powerspec.c:32-90) ----------------
 def _pk_bins_numpy(N, L, bins):
     """The reference's bin formula restated with numpy (u and (bins-1)*u in float32)."""
