@@ -867,9 +867,10 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
 // window (not the case for lattice sites) reads the grid too; the weights,
 // taps and order of the sums are those of tsc_halo<REF>, so the positions
 // are the same bits as k_displace_lattice's.  N must be a multiple of TXN,
-// TY and TZ.
+// TY and TZ.  Four waves per SIMD: the kernel fits 128 registers without
+// spills (168 when the compiler is left free, three work-groups per CU).
 template <int TXN, int TY, int TZ, int PP, int MAXR, int U>
-__global__ void __launch_bounds__(TXN / PP * TY * TZ)
+__global__ void __launch_bounds__(TXN / PP * TY * TZ) __attribute__((amdgpu_waves_per_eu(4)))
     k_displace_tile_ref(const double* __restrict__ px, const double* __restrict__ py,
                         const double* __restrict__ pz, unsigned rs, int N, float spacing,
                         double boxlen, float* __restrict__ X, float* __restrict__ Y,

@@ -840,8 +840,8 @@ struct Pipe {
                                                psi(0), psi(1), psi(2), rs, NN, spacing, L, X,
                                                Y, Z);
                         };
-                        // 256 window rows (37 KB of LDS, three work-groups per CU at
-                        // 168 registers) and eight rows in flight per 32 lanes
+                        // 256 window rows (37 KB of LDS) and eight rows in flight per
+                        // 32 lanes; four work-groups per CU (128 registers)
                         tile(k_displace_tile_ref<8, 8, 16, 4, 256, 8>, 8, 8, 16, 4);
                       }
                     } else {
