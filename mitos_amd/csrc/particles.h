@@ -1647,9 +1647,10 @@ __device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b :
 // the grid (every grid cell lies in exactly one output tile).  Each window
 // cell lies in one output tile, and with R + P <= N per axis only one
 // periodic image of a window meets it, so every cell is counted once; the
-// sums are integers, so the order of the windows does not matter.
+// sums are integers, so the order of the windows does not matter.  Eight
+// waves per SIMD (64 registers, no spills): four work-groups per CU.
 template <int PX, int PY, int PZ, int RX, int RY, int RZ, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8)))
     k_cic_gather(const unsigned* __restrict__ scratch, const int* __restrict__ means,
                  const int* __restrict__ mmax, int N, unsigned long long* __restrict__ grid) {
     static_assert(NT == PY * PZ, "one thread per (y, z) column of the tile");
