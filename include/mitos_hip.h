@@ -505,7 +505,9 @@ int mitos_ic_profile(mitos_ic_context *ctx, int enable);
 int mitos_ic_profile_entry(mitos_ic_context *ctx, int i, const char **name, float *ms,
                            double *bytes);
 
-/* the reference's call order through the Part 1 functions (FP64 plain path) */
+/* the reference's call order through the Part 1 functions (FP64 plain path);
+ * X, Y, Z: optional device arrays of M^3 floats (M^3 = N^3 unless cfg->M is
+ * set) that receive the particle positions, or NULL */
 int mitos_ic_run_reference_order(const struct mitos_ic_config *cfg,
                                  struct mitos_ic_result *res, float *X, float *Y, float *Z);
 
