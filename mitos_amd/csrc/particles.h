@@ -669,7 +669,11 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
     __shared__ unsigned roff[MAXR];      // grid offsets of the z move's window rows
     __shared__ int redx[NW][2], redy[NW][2];
     const int t = threadIdx.x;
-    const int zl = t % TZ, yl = (t / TZ) % TY, xh = t / (TY * TZ);
+    const int zl = t % TZ, yl = (t / TZ) % TY;
+    // x group of the thread: a whole number of waves (TY * TZ a multiple of
+    // 64), so wave-uniform and kept in a scalar register
+    static_assert((TY * TZ) % 64 == 0, "an x group is whole waves");
+    const int xh = __builtin_amdgcn_readfirstlane(t / (TY * TZ));
     const unsigned nzt = (unsigned)N / TZ, nxt = (unsigned)N / TXN, nyt = (unsigned)N / TY;
     const unsigned bid = xcd_block(blockIdx.x, gridDim.x);
     const unsigned zt = bid % nzt, r1 = bid / nzt;
@@ -724,16 +728,15 @@ __global__ void __launch_bounds__(TXN / PP * TY * TZ)
     if (stage_x) {
         // window row xi (x = lo - 1 + xi) of this thread's (y, z) column at
         // buf[(xi * TY + yl) * TZ + zl]; the thread loads rows xh, xh + XG,
-        // ...; its wrapped x follows incrementally (every wrapped row is in
-        // the grid, so the loads past the window need no guard and are not
-        // stored)
+        // ...; its wrapped x follows incrementally
         const unsigned col = (unsigned)qy * rs + qz;
         int xw = wrapf(lo - 1 + xh, N, invN);
         for (int k0 = xh; k0 < nx; k0 += XG * U) {
             T v[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                v[u] = sy[__umul24((unsigned)xw, plane) + col];
+                // rows past the window are not loaded (a wave-uniform test)
+                if (k0 + XG * u < nx) v[u] = sy[__umul24((unsigned)xw, plane) + col];
                 xw += XG;
                 if (xw >= N) xw -= N;
             }
