@@ -254,7 +254,11 @@ struct MultiK<T, L, AXIS, DIR, NIN, CODES, Radices<Rs...>> {
     static int go(const MultiPtrs<T, NIN, CODES::n>& P, long hs, double dk, hipStream_t s) {
         constexpr int C = HeavyCfg<T, L>::C;
         constexpr int E = HeavyCfg<T, L>::E;
-        constexpr bool SPLIT = 2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
+        // one exchange buffer per component unless that leaves less than two
+        // work-groups per CU; the wide form runs one per CU by its registers
+        // anyway (1024 threads at about 124), so it takes both (139 KB)
+        constexpr bool SPLIT = !HeavyCfg<T, L>::WIDE &&
+                               2 * LdsMap<T, L, C, true>::SIZE * sizeof(T) > 81920;
         const int ncols = L / 2 + 1;
         const cx<T>* tw = twiddles<T>(L);
         if (!tw) return MITOS_ERR_ALLOC;
