@@ -815,9 +815,10 @@ struct Pipe {
                                                psi(0), psi(1), psi(2), rs, NN, (T)L, spacing, L,
                                                X, Y, Z, tsc_staged ? 1 : 0);
                         };
-                        // 8 x 8 x 16 tiles, four particles per thread: the window
-                        // limits leave about 0.2% of the tiles of a 1.95 Mpc/h cell
-                        // at z = 0 amplitude to the direct reads
+                        // 8 x 8 x 16 tiles, four particles per thread; windows of at
+                        // most 40 x rows and 576 rows of 16 z (a limit of 608 left
+                        // 0.2% of the tiles of a 1.95 Mpc/h cell at z = 0 amplitude
+                        // to the direct reads), four work-groups per CU
                         tile(k_displace_tile<T, 8, 8, 16, 4, 40, 576, 12, 0>, 8, 8, 16, 4);
                     } else if (STENCIL && NN % 64 == 0) {
                         constexpr int TX = 16, PPT = 8;
